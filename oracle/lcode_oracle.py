@@ -1,0 +1,565 @@
+"""Float64 CPU oracle for lcodePy's 3D quasistatic xi step.
+
+TEST INFRASTRUCTURE ONLY.  This module is the checker for the HIP product
+path in ``lcodepy_amd``: only ``tests/``, ``__graft_entry__.smoke()`` and
+``bench.py``'s ``cpu_baseline`` leg may import it, and only to compare or to
+time.  The product never imports, links or falls back to it.
+
+It restates, in numpy plus the C kernels of ``oracle/kernels.c``:
+
+* the plasma lattice            -- plasma3d/initialization.py:72-135
+* the solver matrices           -- plasma3d/initialization.py:15-67
+* the background-ion density    -- plasma3d/initialization.py:351-356
+* compute_rhoj                  -- plasma3d/rhoj.py:11-20, weights.py:249-285
+* compute_fields                -- plasma3d/fields.py:112-322 (CPU branch,
+                                   scipy.fftpack DST-I/DCT-I as fields.py:15-31)
+* step_dxi                      -- plasma3d/solver.py:25-78
+* the beam layer bookkeeping    -- beam3d/beam_calculator.py:60-152,
+                                   beam3d/beam_io.py:7-66, beam3d/data.py:88-169
+* the xi loop                   -- push_solvers/push_solver.py:215-286
+* the time loop                 -- simulation.py:244-264, mpi/beam_io.py:81-119
+
+Parity pins (tests/test_oracle_*.py): the reference's own end-state fixtures
+``3D_test1.npz`` and ``3D_beam_evol.npz`` (tests/integration/test_3D_part.py,
+rtol 5e-8), closed-form eigenmodes of the three solves, DST-I/DCT-I
+self-inverse identities, spline-weight invariants and single-particle cases.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from dataclasses import dataclass, field
+
+import numpy as np
+from scipy.fftpack import dct, dctn, dst, dstn
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_i64p = ctypes.POINTER(ctypes.c_int64)
+_u8p = ctypes.POINTER(ctypes.c_uint8)
+
+
+def lib():
+    """Load (building on first use when a compiler is present) liboracle.so."""
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(_HERE, 'liboracle.so')
+        src = os.path.join(_HERE, 'kernels.c')
+        if not os.path.exists(path) or os.path.getmtime(path) < os.path.getmtime(src):
+            import subprocess
+            subprocess.run(['make', '-C', _HERE, '-s'], check=True)
+        _LIB = ctypes.CDLL(path)
+        _LIB.oracle_weight4.restype = ctypes.c_double
+        _LIB.oracle_weight4.argtypes = [ctypes.c_double, ctypes.c_int]
+    return _LIB
+
+
+def _p(a: np.ndarray):
+    assert a.flags.c_contiguous, 'oracle arrays must be C-contiguous'
+    if a.dtype == np.float64:
+        return a.ctypes.data_as(_dp)
+    if a.dtype == np.int64:
+        return a.ctypes.data_as(_i64p)
+    if a.dtype == np.uint8:
+        return a.ctypes.data_as(_u8p)
+    raise TypeError(a.dtype)
+
+
+def weight4(x: float, place: int) -> float:
+    """Quartic spline weight, plasma3d/weights.py:12-31."""
+    return lib().oracle_weight4(float(x), int(place))
+
+
+# --------------------------------------------------------------------------
+# Parameters
+# --------------------------------------------------------------------------
+@dataclass
+class Params:
+    N: int                      # window-width-steps (odd)
+    h: float                    # transverse-step
+    xi_step: float
+    subtraction_coeff: float
+    bound_padding_steps: int = 10
+    plasma_padding_steps: int = 15
+    fineness: int = 2
+    ion_model: str = 'background'
+    ion_mass: float = 1836.0
+    window_length: float = 10.0
+    time_step: float = 1.0
+    substepping_energy: float = 2.0
+    nthreads: int = 0           # 0: all OpenMP threads
+
+    @property
+    def boundary(self) -> float:   # plasma3d/move.py:321
+        return self.h * (self.N / 2 - self.bound_padding_steps)
+
+    @property
+    def lost_radius(self) -> float:  # beam3d/move.py:285-286
+        max_radius = self.h * self.N / 2
+        return max(0.9 * max_radius, max_radius - 1)
+
+    @property
+    def xi_steps(self) -> int:       # push_solver.py:206
+        return round(self.window_length / self.xi_step)
+
+    @classmethod
+    def from_config(cls, config, nthreads: int = 0) -> 'Params':
+        """From an adjusted lcodepy_amd Config (same keys as the reference)."""
+        fin = config.getfloat('plasma-fineness')
+        return cls(N=config.getint('window-width-steps'),
+                   h=config.getfloat('transverse-step'),
+                   xi_step=config.getfloat('xi-step'),
+                   subtraction_coeff=config.getfloat('field-solver-subtraction-coefficient'),
+                   bound_padding_steps=config.getint('bound-padding-steps'),
+                   plasma_padding_steps=config.getint('plasma-padding-steps'),
+                   fineness=int(fin) if fin > 1 else fin,
+                   ion_model=config.get('ion-model'),
+                   ion_mass=config.getfloat('ion-mass'),
+                   window_length=config.getfloat('window-length'),
+                   time_step=config.getfloat('time-step'),
+                   substepping_energy=config.getfloat('beam-substepping-energy'),
+                   nthreads=nthreads)
+
+
+# --------------------------------------------------------------------------
+# Initial state
+# --------------------------------------------------------------------------
+def plasma_grid(steps: int, step_size: float, fineness) -> np.ndarray:
+    """1-D lattice of initial coordinates, initialization.py:72-104."""
+    plasma_step = step_size / fineness
+    if fineness % 2:
+        right = np.arange(steps // 2 * fineness) * plasma_step
+        left = -right[:0:-1]
+    elif 0. < fineness < 1.:
+        right = np.arange(steps // (2 / fineness)) * plasma_step
+        left = -right[:0:-1]
+    else:
+        right = (.5 + np.arange(steps // 2 * fineness)) * plasma_step
+        left = -right[::-1]
+    return np.concatenate([left, right])
+
+
+def uniform_plasma(p: Params) -> dict:
+    """Uniform electron lattice with zero momenta, initialization.py:107-135
+    (uniform density: the profile coefficient is 1)."""
+    g = plasma_grid(p.N - 2 * p.plasma_padding_steps, p.h, p.fineness)
+    n = len(g)
+    y_init = np.ascontiguousarray(np.broadcast_to(g, (n, n)))
+    x_init = y_init.T.copy()
+    q = np.ones_like(x_init) * -1 / p.fineness ** 2 * p.h ** 2
+    m = np.ones_like(x_init) * 1 / p.fineness ** 2 * p.h ** 2
+    z = np.zeros_like(x_init)
+    return dict(x_init=x_init, y_init=y_init, x_offt=z.copy(), y_offt=z.copy(),
+                px=z.copy(), py=z.copy(), pz=z.copy(), q=q, m=m)
+
+
+def ions_like(electrons: dict, ion_mass: float) -> dict:
+    """Mobile ion sort, initialization.py:395-407."""
+    ions = {k: v.copy() for k, v in electrons.items()}
+    ions['q'] = -electrons['q'].copy()
+    ions['m'] = ion_mass * electrons['m'].copy()
+    return ions
+
+
+def eigenvalues(N: int, h: float) -> np.ndarray:
+    """lambda_k = 4/h^2 sin^2(k pi / (2(N-1))), k = 0..N-1 (initialization.py:24)."""
+    k = np.arange(N)
+    return 4 / h ** 2 * np.sin(k * np.pi / (2 * (N - 1))) ** 2
+
+
+def dirichlet_matrix(N: int, h: float) -> np.ndarray:
+    """initialization.py:15-27."""
+    k = np.arange(1, N - 1)
+    lam = 4 / h ** 2 * np.sin(k * np.pi / (2 * (N - 1))) ** 2
+    mul = 1 / (lam[:, None] + lam[None, :])
+    return mul / (2 * (N - 1)) ** 2
+
+
+def mixed_matrix(N: int, h: float, c: float) -> np.ndarray:
+    """initialization.py:32-47."""
+    ki, kj = np.arange(1, N - 1), np.arange(N)
+    li = 4 / h ** 2 * np.sin(ki * np.pi / (2 * (N - 1))) ** 2
+    lj = 4 / h ** 2 * np.sin(kj * np.pi / (2 * (N - 1))) ** 2
+    mul = 1 / (li[:, None] + lj[None, :] + c)
+    return mul / (2 * (N - 1)) ** 2
+
+
+def neumann_matrix(N: int, h: float) -> np.ndarray:
+    """initialization.py:53-67."""
+    k = np.arange(0, N)
+    lam = 4 / h ** 2 * np.sin(k * np.pi / (2 * (N - 1))) ** 2
+    with np.errstate(divide='ignore'):
+        mul = 1 / (lam[:, None] + lam[None, :])
+    mul[0, 0] = 0
+    return mul / (2 * (N - 1)) ** 2
+
+
+@dataclass
+class Const:
+    ro_initial: np.ndarray
+    dirichlet: np.ndarray
+    mixed: np.ndarray
+    neumann: np.ndarray
+    sorts: dict = field(default_factory=lambda: {'electrons': 0})
+
+
+def zero_fields(N: int) -> dict:
+    return {k: np.zeros((N, N)) for k in ('Ex', 'Ey', 'Ez', 'Bx', 'By', 'Bz', 'Phi')}
+
+
+def zero_currents(N: int) -> dict:
+    return {k: np.zeros((2, N, N)) for k in ('ro', 'jx', 'jy', 'jz')}
+
+
+def init_plasma(p: Params, electrons: dict | None = None):
+    """initialization.py:278-414 for the single-plasma approach.  Returns
+    (fields, particles, currents, const, xi_plasma_layer)."""
+    if p.N % 2 != 1:
+        raise AssertionError('grid_steps must be odd')
+    if p.bound_padding_steps <= 2:
+        raise Exception("'bound_padding_steps' parameter is too low.")
+    el = uniform_plasma(p) if electrons is None else {k: v.copy() for k, v in electrons.items()}
+    particles = {'electrons': el}
+    ro0 = np.zeros((p.N, p.N))
+    z = np.zeros_like(ro0)
+    deposit_sort(p, el, ro0, z.copy(), z.copy(), z.copy())
+    const = Const(ro_initial=-ro0, dirichlet=dirichlet_matrix(p.N, p.h),
+                  mixed=mixed_matrix(p.N, p.h, p.subtraction_coeff),
+                  neumann=neumann_matrix(p.N, p.h))
+    if p.ion_model == 'mobile':
+        particles['ions'] = ions_like(el, p.ion_mass)
+        const.sorts = {'electrons': 0, 'ions': 1}
+    return zero_fields(p.N), particles, zero_currents(p.N), const, p.xi_step
+
+
+# --------------------------------------------------------------------------
+# Particle kernels (C)
+# --------------------------------------------------------------------------
+def move_wo_fields(p: Params, particles: dict) -> dict:
+    """plasma3d/move.py:331-349: deep copy, then free-stream each sort."""
+    full = {s: {k: v.copy() for k, v in arrs.items()} for s, arrs in particles.items()}
+    for arrs in full.values():
+        lib().oracle_move_wo_fields(ctypes.c_int64(arrs['m'].size), ctypes.c_double(p.xi_step),
+                                    _p(arrs['m']), _p(arrs['x_offt']), _p(arrs['y_offt']),
+                                    _p(arrs['px']), _p(arrs['py']), _p(arrs['pz']),
+                                    ctypes.c_int(p.nthreads))
+    return full
+
+
+def move_smart(p: Params, fields: dict, prev: dict, full: dict) -> dict:
+    """plasma3d/move.py:351-381 (in place on ``full``)."""
+    for sort in prev:
+        a, b = prev[sort], full[sort]
+        lib().oracle_move_smart(
+            ctypes.c_int64(a['m'].size), ctypes.c_double(p.xi_step),
+            ctypes.c_double(p.boundary), ctypes.c_double(p.h), ctypes.c_int(p.N),
+            _p(fields['Ex']), _p(fields['Ey']), _p(fields['Ez']),
+            _p(fields['Bx']), _p(fields['By']), _p(fields['Bz']),
+            _p(a['x_init']), _p(a['y_init']), _p(a['x_offt']), _p(a['y_offt']),
+            _p(a['px']), _p(a['py']), _p(a['pz']),
+            _p(b['m']), _p(b['q']), _p(b['x_offt']), _p(b['y_offt']),
+            _p(b['px']), _p(b['py']), _p(b['pz']), ctypes.c_int(p.nthreads))
+    return full
+
+
+def deposit_sort(p: Params, a: dict, ro, jx, jy, jz, nthreads: int | None = None):
+    """plasma3d/weights.py:35-67 into (N, N) views (accumulating)."""
+    lib().oracle_deposit_plasma(
+        ctypes.c_int64(a['m'].size), ctypes.c_int(p.N), ctypes.c_double(p.h),
+        _p(a['x_init']), _p(a['y_init']), _p(a['x_offt']), _p(a['y_offt']),
+        _p(a['px']), _p(a['py']), _p(a['pz']), _p(a['q']), _p(a['m']),
+        _p(ro), _p(jx), _p(jy), _p(jz),
+        ctypes.c_int(p.nthreads if nthreads is None else nthreads))
+
+
+def compute_rhoj(p: Params, particles: dict, const: Const, nthreads: int | None = None) -> dict:
+    """plasma3d/rhoj.py:11-20 with weights.py:249-285."""
+    cur = zero_currents(p.N)
+    for sort, col in const.sorts.items():
+        deposit_sort(p, particles[sort], cur['ro'][col], cur['jx'][col],
+                     cur['jy'][col], cur['jz'][col], nthreads)
+    if p.ion_model == 'background':
+        cur['ro'][1, :, :] = const.ro_initial
+    return cur
+
+
+# --------------------------------------------------------------------------
+# Field solves (numpy + scipy.fftpack, fields.py CPU branch)
+# --------------------------------------------------------------------------
+def dst2d(a):
+    return dstn(a, type=1)
+
+
+def dct2d(a):
+    return dctn(a, type=1)
+
+
+def mix2d(a):
+    """DST-I along axis 0, DCT-I along axis 1, zero rows padded (fields.py:19-31)."""
+    out = dct(dst(a, type=1, axis=0), type=1, axis=1)
+    return np.pad(out, ((1, 1), (0, 0)), 'constant', constant_values=0)
+
+
+def calculate_Ez(h, const: Const, cur):
+    """fields.py:112-139."""
+    jx, jy = cur['jx'], cur['jy']
+    djx_dx = jx[2:, 1:-1] - jx[:-2, 1:-1]
+    djy_dy = jy[1:-1, 2:] - jy[1:-1, :-2]
+    rhs_inner = -(djx_dx + djy_dy) / (h * 2)
+    f = dst2d(rhs_inner)
+    f *= const.dirichlet
+    return np.pad(dst2d(f), 1, 'constant', constant_values=0)
+
+
+def calculate_Phi(const: Const, cur):
+    """fields.py:142-159."""
+    rhs_inner = (cur['ro'] - cur['jz'])[1:-1, 1:-1]
+    f = dst2d(rhs_inner)
+    f *= const.dirichlet
+    return np.pad(dst2d(f), 1, 'constant', constant_values=0)
+
+
+def dx_dy(arr, h):
+    """fields.py:164-173."""
+    dx, dy = np.zeros_like(arr), np.zeros_like(arr)
+    dx[1:-1, 1:-1] = arr[2:, 1:-1] - arr[:-2, 1:-1]
+    dy[1:-1, 1:-1] = arr[1:-1, 2:] - arr[1:-1, :-2]
+    return dx / (h * 2), dy / (h * 2)
+
+
+def calculate_Ex_Ey_Bx_By(p: Params, const: Const, fields, ro_beam_full, ro_beam_prev,
+                          cur_full, cur_prev):
+    """fields.py:176-230."""
+    ro_half = (cur_full['ro'] + ro_beam_full + cur_prev['ro'] + ro_beam_prev) / 2
+    jz_half = (cur_full['jz'] + ro_beam_full + cur_prev['jz'] + ro_beam_prev) / 2
+    dro_dx, dro_dy = dx_dy(ro_half, p.h)
+    djz_dx, djz_dy = dx_dy(jz_half, p.h)
+    djx_dxi = (cur_prev['jx'] - cur_full['jx']) / p.xi_step
+    djy_dxi = (cur_prev['jy'] - cur_full['jy']) / p.xi_step
+    c = p.subtraction_coeff
+    Ex_rhs = -(dro_dx - djx_dxi - c * fields['Ex'])
+    Ey_rhs = -(dro_dy - djy_dxi - c * fields['Ey'])
+    Bx_rhs = +(djz_dy - djy_dxi + c * fields['Bx'])
+    By_rhs = -(djz_dx - djx_dxi - c * fields['By'])
+    mm = const.mixed
+    Ey_f = mix2d(Ey_rhs[1:-1, :])[1:-1, :]
+    Ey_f *= mm
+    Ey = mix2d(Ey_f)
+    Bx = mix2d(mm * mix2d(Bx_rhs[1:-1, :])[1:-1, :])
+    By = mix2d(mm * mix2d(By_rhs.T[1:-1, :])[1:-1, :]).T
+    Ex = mix2d(mm * mix2d(Ex_rhs.T[1:-1, :])[1:-1, :]).T
+    return Ex, Ey, Bx, By
+
+
+def calculate_Bz(h, const: Const, cur):
+    """fields.py:233-266."""
+    jx, jy = cur['jx'], cur['jy']
+    djx_dy = np.pad(jx[1:-1, 2:] - jx[1:-1, :-2], 1, 'constant', constant_values=0)
+    djy_dx = np.pad(jy[2:, 1:-1] - jy[:-2, 1:-1], 1, 'constant', constant_values=0)
+    rhs = -(djx_dy - djy_dx) / (h * 2)
+    f = dct2d(rhs)
+    f *= const.neumann
+    Bz = dct2d(f)
+    Bz -= Bz.mean()
+    return Bz
+
+
+def compute_fields(p: Params, fields, fields_prev, const: Const, ro_beam_full, ro_beam_prev,
+                   currents_prev, currents_full):
+    """fields.py:276-322: returns (fields_full, fields_half)."""
+    cf = {k: np.sum(currents_full[k], axis=0) for k in ('ro', 'jx', 'jy', 'jz')}
+    cp = {k: np.sum(currents_prev[k], axis=0) for k in ('ro', 'jx', 'jy', 'jz')}
+    Ex_h, Ey_h, Bx_h, By_h = calculate_Ex_Ey_Bx_By(p, const, fields, ro_beam_full,
+                                                   ro_beam_prev, cf, cp)
+    Ex_f = 2 * Ex_h - fields_prev['Ex']
+    Ey_f = 2 * Ey_h - fields_prev['Ey']
+    Bx_f = 2 * Bx_h - fields_prev['Bx']
+    By_f = 2 * By_h - fields_prev['By']
+    Ez_f = calculate_Ez(p.h, const, cf)
+    Bz_f = calculate_Bz(p.h, const, cf)
+    Phi = calculate_Phi(const, cf)
+    full = dict(Ex=Ex_f, Ey=Ey_f, Ez=Ez_f, Bx=Bx_f, By=By_f, Bz=Bz_f, Phi=Phi)
+    half = dict(Ex=Ex_h, Ey=Ey_h, Ez=(Ez_f + fields_prev['Ez']) / 2,
+                Bx=Bx_h, By=By_h, Bz=(Bz_f + fields_prev['Bz']) / 2, Phi=Phi)
+    # Arrays() copies (plasma3d/data.py:23); the C kernels want row-major.
+    full = {k: np.ascontiguousarray(v) for k, v in full.items()}
+    half = {k: np.ascontiguousarray(v) for k, v in half.items()}
+    return full, half
+
+
+def step_dxi(p: Params, particles_prev, fields_prev, currents_prev, const: Const,
+             ro_beam_full, ro_beam_prev, trace: dict | None = None):
+    """plasma3d/solver.py:25-78 (declustering off).  If ``trace`` is a dict
+    the intermediate stages are stored in it for stage-by-stage checks."""
+    full = move_wo_fields(p, particles_prev)
+    if trace is not None:
+        trace['free'] = {s: {k: v.copy() for k, v in a.items()} for s, a in full.items()}
+    full = move_smart(p, fields_prev, particles_prev, full)
+    currents_full = compute_rhoj(p, full, const)
+    if trace is not None:
+        trace['push1'] = {s: {k: v.copy() for k, v in a.items()} for s, a in full.items()}
+        trace['rhoj1'] = {k: v.copy() for k, v in currents_full.items()}
+    _, fields_half = compute_fields(p, fields_prev, fields_prev, const, ro_beam_full,
+                                    ro_beam_prev, currents_prev, currents_full)
+    if trace is not None:
+        trace['half1'] = dict(fields_half)
+    full = move_smart(p, fields_half, particles_prev, full)
+    currents_full = compute_rhoj(p, full, const)
+    if trace is not None:
+        trace['push2'] = {s: {k: v.copy() for k, v in a.items()} for s, a in full.items()}
+        trace['rhoj2'] = {k: v.copy() for k, v in currents_full.items()}
+    fields_full, fields_half = compute_fields(p, fields_half, fields_prev, const, ro_beam_full,
+                                              ro_beam_prev, currents_prev, currents_full)
+    if trace is not None:
+        trace['half2'] = dict(fields_half)
+        trace['full2'] = dict(fields_full)
+    full = move_smart(p, fields_half, particles_prev, full)
+    currents_full = compute_rhoj(p, full, const)
+    return full, fields_full, currents_full
+
+
+# --------------------------------------------------------------------------
+# Beam (CPU kernels of beam3d) and the xi / time loops
+# --------------------------------------------------------------------------
+BEAM_KEYS = ('xi', 'x', 'y', 'px', 'py', 'pz', 'q_m', 'q_norm', 'id', 'dt', 'remaining_steps')
+
+
+def beam_from_records(rec: np.ndarray) -> dict:
+    """BeamParticles.init_generated (beam3d/data.py:48-61)."""
+    b = {k: np.array(rec[k]) for k in ('xi', 'x', 'y', 'px', 'py', 'pz', 'q_m', 'q_norm', 'id')}
+    b['dt'] = np.zeros_like(b['q_norm'])
+    b['remaining_steps'] = np.zeros_like(b['id'])
+    return b
+
+
+def empty_beam() -> dict:
+    b = {k: np.zeros(0) for k in BEAM_KEYS}
+    b['id'] = np.zeros(0, dtype=np.int64)
+    b['remaining_steps'] = np.zeros(0, dtype=np.int64)
+    return b
+
+
+def beam_sorted(b: dict) -> dict:
+    """beam3d/data.py:88-104."""
+    idx = np.argsort(-b['xi'])
+    return {k: v[idx] for k, v in b.items()}
+
+
+def beam_take(b: dict, sel) -> dict:
+    return {k: np.ascontiguousarray(v[sel]) for k, v in b.items()}
+
+
+def beam_append(a: dict, b: dict) -> dict:
+    return {k: np.concatenate((a[k], b[k])) for k in BEAM_KEYS}
+
+
+class BeamCalc:
+    """BeamCalculator (beam3d/beam_calculator.py:60-152) with CPU kernels."""
+
+    def __init__(self, p: Params):
+        self.p = p
+        self.rho_layout = np.zeros((p.N, p.N))
+
+    def start_time_step(self):
+        self.rho_layout = np.zeros((self.p.N, self.p.N))
+
+    def layout_beam_layer(self, layer: dict, plasma_layer_idx: int):
+        p = self.p
+        rho = np.zeros_like(self.rho_layout)
+        if layer['id'].size != 0:
+            lib().oracle_deposit_beam(
+                ctypes.c_int64(layer['id'].size), ctypes.c_int(p.N), ctypes.c_double(p.h),
+                ctypes.c_double(p.xi_step), ctypes.c_int64(plasma_layer_idx),
+                _p(layer['x']), _p(layer['y']), _p(layer['xi']), _p(layer['q_norm']),
+                _p(self.rho_layout), _p(rho), ctypes.c_int(p.nthreads))
+        self.rho_layout, rho = rho, self.rho_layout
+        rho /= p.h ** 2
+        return rho
+
+    def move_beam_layer(self, layer: dict, fell_size: int, pl_layer_idx: int, f_k, f_k1):
+        p = self.p
+        n = layer['id'].size
+        lost = np.zeros(n, dtype=np.uint8)
+        moved = np.zeros(n, dtype=np.uint8)
+        fell = np.zeros(n, dtype=np.uint8)
+        if n != 0:
+            size = n - fell_size
+            dt = np.zeros(size)
+            qm = np.ascontiguousarray(layer['q_m'][:size])
+            pz = np.ascontiguousarray(layer['pz'][:size])
+            lib().oracle_beam_substep(ctypes.c_int64(size), _p(qm), _p(pz),
+                                      ctypes.c_double(p.substepping_energy), _p(dt))
+            layer['dt'][:size] = dt * p.time_step
+            layer['remaining_steps'][:size] = (1. / dt).astype(np.int_)
+            lib().oracle_move_beam(
+                ctypes.c_int64(n), ctypes.c_double(p.xi_step), ctypes.c_double(p.lost_radius),
+                ctypes.c_int64(pl_layer_idx - 1), ctypes.c_double(p.h), ctypes.c_int(p.N),
+                _p(layer['q_m']), _p(layer['dt']),
+                _p(f_k['Ex']), _p(f_k['Ey']), _p(f_k['Ez']),
+                _p(f_k['Bx']), _p(f_k['By']), _p(f_k['Bz']),
+                _p(f_k1['Ex']), _p(f_k1['Ey']), _p(f_k1['Ez']),
+                _p(f_k1['Bx']), _p(f_k1['By']), _p(f_k1['Bz']),
+                _p(layer['remaining_steps']), _p(layer['id']),
+                _p(layer['x']), _p(layer['y']), _p(layer['xi']),
+                _p(layer['px']), _p(layer['py']), _p(layer['pz']),
+                _p(lost), _p(moved), _p(fell), ctypes.c_int(p.nthreads))
+        return (beam_take(layer, lost.astype(bool)), beam_take(layer, moved.astype(bool)),
+                beam_take(layer, fell.astype(bool)))
+
+
+def layer_to_layout(p: Params, source: dict, plasma_layer_idx: int):
+    """BeamSource.get_beam_layer_to_layout + cut_beam_layer (beam_io.py:29-47,
+    data.py:128-149).  Returns (layer, remaining_source)."""
+    xi_min = -p.xi_step * plasma_layer_idx
+    xi_max = -p.xi_step * (plasma_layer_idx + 1)
+    xi = source['xi']
+    n = int(np.sum((xi_max <= xi) * (xi < xi_min))) if xi.size else 0
+    return ({k: v[:n] for k, v in source.items()}, {k: v[n:] for k, v in source.items()})
+
+
+def step_dt(p: Params, fields, particles, currents, const: Const, xi_plasma_layer_start,
+            source: dict, on_step=None):
+    """PusherAndSolver3D.step_dt (push_solver.py:215-286).  Returns
+    (particles, fields, currents, drained_beam, on_axis_Ez list)."""
+    beam_calc = BeamCalc(p)
+    beam_calc.start_time_step()
+    layer_to_move = empty_beam()
+    fell_size = 0
+    ro_beam_prev = np.zeros((p.N, p.N))
+    drain = empty_beam()
+    ez_axis = []
+    start = round(-xi_plasma_layer_start / p.xi_step) + 1
+    for xi_i in range(start, p.xi_steps + 1):
+        layer, source = layer_to_layout(p, source, xi_i)
+        ro_beam_full = beam_calc.layout_beam_layer(layer, xi_i)
+        prev_fields = {k: v.copy() for k, v in fields.items()}
+        particles, fields, currents = step_dxi(p, particles, fields, currents, const,
+                                               ro_beam_full, ro_beam_prev)
+        lost, moved, fell = beam_calc.move_beam_layer(layer_to_move, fell_size, xi_i,
+                                                      prev_fields, fields)
+        layer_to_move = beam_append(layer, fell)
+        fell_size = fell['id'].size
+        ro_beam_prev = ro_beam_full.copy()
+        if moved['id'].size > 0:
+            drain = beam_append(drain, moved)
+        ez_axis.append(fields['Ez'][p.N // 2, p.N // 2])
+        if on_step is not None:
+            on_step(xi_i, particles, fields, currents, ro_beam_full)
+    return particles, fields, currents, drain, np.array(ez_axis)
+
+
+def simulate(p: Params, beam_records: np.ndarray, n_time_steps: int, electrons=None):
+    """Simulation.step for a single process (simulation.py:244-264 with
+    mpi/beam_io.py:81-119): the plasma is re-initialised every time step and
+    the drained beam becomes the next source.  Returns the final
+    (particles, fields, currents, beam, ez_axis)."""
+    source = beam_sorted(beam_from_records(beam_records))
+    out = None
+    for t_i in range(n_time_steps):
+        fields, particles, currents, const, xi0 = init_plasma(p, electrons)
+        out = step_dt(p, fields, particles, currents, const, xi0, source)
+        if t_i != n_time_steps - 1:
+            source = beam_sorted(out[3])
+    return out

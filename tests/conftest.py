@@ -1,0 +1,20 @@
+import os
+import sys
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+GOLDEN = os.path.join(ROOT, 'tests', 'golden')
+
+
+def pytest_configure(config):
+    config.addinivalue_line('markers', 'gpu: needs an MI355X (runs the HIP path)')
+    config.addinivalue_line('markers', 'slow: end-to-end oracle runs (tens of seconds)')
+
+
+@pytest.fixture(scope='session')
+def golden_dir():
+    return GOLDEN

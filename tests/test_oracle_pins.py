@@ -1,0 +1,313 @@
+"""Independent pins of each oracle function (what the fixtures leave open).
+
+Every check is a mathematical fact or a special case that a plausible slip
+(dropped term, wrong sign, swapped axis, wrong normalisation) would break:
+discrete-operator residuals of the three solves, the cardinal quartic
+B-spline, charge conservation, single-particle motion, and bit-identity of
+the banded-parallel deposit with the serial loop order."""
+import math
+
+import numpy as np
+import pytest
+
+from lcodepy_amd.config import Config, choose_grid_steps, good_size
+from oracle import lcode_oracle as O
+
+
+# -- config (config.py:185-258) ----------------------------------------------
+@pytest.mark.parametrize('width,step,n', [(16, 0.05, 321), (12.8, 0.1, 127), (12.9, 0.1, 129),
+                                          (25.6, 0.05, 513), (25.6, 0.025, 1025),
+                                          (30.72, 0.04, 769)])
+def test_grid_size_choice(width, step, n):
+    assert choose_grid_steps(width, step) == n
+    assert good_size(n)
+
+
+def test_good_size_rules():
+    assert good_size(513) and good_size(769) and good_size(385)
+    assert not good_size(512)            # even
+    assert not good_size(2 * 11 * 13 + 1)  # two of {11, 13}
+    assert not good_size(17 * 2 + 1)     # factor 17
+
+
+@pytest.mark.parametrize('ppc,fin,ppc_out', [(1, 1, 1), (2, 1, 1), (3, 2, 4), (4, 2, 4),
+                                             (9, 3, 9), (10, 3, 9)])
+def test_fineness_rounding(ppc, fin, ppc_out):
+    c = Config({'geometry': '3d', 'plasma-particles-per-cell': ppc})
+    c._adjust_config_values_3d()
+    assert c.getint('plasma-fineness') == fin
+    assert c.getint('plasma-particles-per-cell') == ppc_out
+
+
+@pytest.mark.parametrize('value,expected', [(1, 1.05), ('1', 1.05), (1.0, 1.0), (0.5, 0.5)])
+def test_subtraction_coefficient_string_rule(value, expected):
+    c = Config({'geometry': '3d', 'xi-step': 0.05,
+                'field-solver-subtraction-coefficient': value})
+    c._adjust_config_values_3d()
+    assert c.getfloat('field-solver-subtraction-coefficient') == pytest.approx(expected, abs=0)
+
+
+# -- quartic spline (weights.py:12-31) ----------------------------------------
+def _cardinal_bspline4(t):
+    """Centred degree-4 cardinal B-spline from the truncated-power formula."""
+    s = t + 2.5
+    return sum((-1) ** j * math.comb(5, j) * max(s - j, 0.0) ** 4 for j in range(6)) / 24.0
+
+
+@pytest.mark.parametrize('x', [-0.5, -0.37, -0.1, 0.0, 0.013, 0.25, 0.4999])
+def test_weight4_is_the_cardinal_bspline(x):
+    w = [O.weight4(x, k) for k in range(-2, 3)]
+    for k, wk in zip(range(-2, 3), w):
+        assert wk == pytest.approx(_cardinal_bspline4(k - x), abs=1e-13)  # formula cancels ~1e-14
+    assert sum(w) == pytest.approx(1.0, abs=1e-15)
+    assert sum(k * wk for k, wk in zip(range(-2, 3), w)) == pytest.approx(x, abs=1e-15)
+    for k in range(-2, 3):
+        assert O.weight4(-x, -k) == pytest.approx(O.weight4(x, k), abs=1e-16)
+
+
+def test_weight4_at_cell_centre():
+    assert [O.weight4(0.0, k) for k in range(-2, 3)] == pytest.approx(
+        [1 / 384, 19 / 96, 115 / 192, 19 / 96, 1 / 384], abs=0)
+
+
+# -- transforms and solves (fields.py) -----------------------------------------
+def _lap(u, h):
+    return (u[2:, 1:-1] + u[:-2, 1:-1] + u[1:-1, 2:] + u[1:-1, :-2] - 4 * u[1:-1, 1:-1]) / h ** 2
+
+
+@pytest.mark.parametrize('N', [9, 17, 33])
+def test_dst_dct_self_inverse(N):
+    rng = np.random.default_rng(N)
+    a = rng.standard_normal((N - 2, N - 2))
+    assert np.allclose(O.dst2d(O.dst2d(a)), (2 * (N - 1)) ** 2 * a, rtol=1e-12, atol=1e-10)
+    b = rng.standard_normal((N, N))
+    assert np.allclose(O.dct2d(O.dct2d(b)), (2 * (N - 1)) ** 2 * b, rtol=1e-12, atol=1e-10)
+
+
+@pytest.mark.parametrize('N,h', [(17, 0.1), (33, 0.05)])
+def test_dirichlet_solve_inverts_the_5point_laplacian(N, h):
+    rng = np.random.default_rng(1)
+    f = rng.standard_normal((N - 2, N - 2))
+    u = np.pad(O.dst2d(O.dst2d(f) * O.dirichlet_matrix(N, h)), 1)
+    assert np.allclose(-_lap(u, h), f, rtol=1e-10, atol=1e-10)
+
+
+@pytest.mark.parametrize('N,h,c', [(17, 0.1, 1.0), (33, 0.05, 1.05)])
+def test_mixed_solve_is_helmholtz_dirichlet_x_neumann_y(N, h, c):
+    rng = np.random.default_rng(2)
+    f = rng.standard_normal((N - 2, N))
+    u = O.mix2d(O.mixed_matrix(N, h, c) * O.mix2d(f)[1:-1, :])
+    assert np.all(u[0] == 0) and np.all(u[-1] == 0)
+    ue = np.concatenate([u[:, 1:2], u, u[:, -2:-1]], axis=1)   # even reflection in y
+    res = -(ue[2:, 1:-1] + ue[:-2, 1:-1] + ue[1:-1, 2:] + ue[1:-1, :-2]
+            - 4 * ue[1:-1, 1:-1]) / h ** 2 + c * u[1:-1, :]
+    assert np.allclose(res, f, rtol=1e-10, atol=1e-10)
+
+
+@pytest.mark.parametrize('N,h', [(17, 0.1), (33, 0.05)])
+def test_neumann_solve_inverts_laplacian_on_the_non_constant_modes(N, h):
+    rng = np.random.default_rng(3)
+    f = rng.standard_normal((N, N))
+    u = O.dct2d(O.dct2d(f) * O.neumann_matrix(N, h))
+    w = np.ones(N)
+    w[0] = w[-1] = 0.5
+    fbar = (w[:, None] * w[None, :] * f).sum() / (w.sum() ** 2)
+    ue = np.pad(u, 1, mode='reflect')
+    assert np.allclose(-_lap(ue, h), f - fbar, rtol=1e-9, atol=1e-9)
+
+
+def test_eigenmode_closed_form():
+    N, h = 33, 0.05
+    i = np.arange(1, N - 1)
+    a, b = 3, 5
+    mode = np.sin(a * np.pi * i / (N - 1))[:, None] * np.sin(b * np.pi * i / (N - 1))[None, :]
+    lam = O.eigenvalues(N, h)
+    u = O.dst2d(O.dst2d(mode) * O.dirichlet_matrix(N, h))
+    assert np.allclose(u, mode / (lam[a] + lam[b]), rtol=1e-12, atol=1e-14)
+
+
+def test_compute_fields_zero_sources_give_zero_fields():
+    p = O.Params(N=17, h=0.1, xi_step=0.05, subtraction_coeff=1.0)
+    const = O.Const(ro_initial=np.zeros((17, 17)), dirichlet=O.dirichlet_matrix(17, 0.1),
+                    mixed=O.mixed_matrix(17, 0.1, 1.0), neumann=O.neumann_matrix(17, 0.1))
+    z = O.zero_fields(17)
+    full, half = O.compute_fields(p, z, z, const, np.zeros((17, 17)), np.zeros((17, 17)),
+                                  O.zero_currents(17), O.zero_currents(17))
+    for k in full:
+        assert not np.any(full[k]) and not np.any(half[k])
+
+
+# -- deposit (weights.py:35-67) -------------------------------------------------
+def _one_particle(x, y, px=0.0, py=0.0, pz=0.0, q=-0.0025, m=0.0025):
+    f = lambda v: np.array([[v]], dtype=np.float64)
+    return dict(x_init=f(x), y_init=f(y), x_offt=f(0.0), y_offt=f(0.0), px=f(px), py=f(py),
+                pz=f(pz), q=f(q), m=f(m))
+
+
+def test_deposit_single_particle_at_cell_centre():
+    N, h = 17, 0.1
+    p = O.Params(N=N, h=h, xi_step=0.05, subtraction_coeff=1.0)
+    a = _one_particle(0.2, -0.3)     # centre of cell (ix, iy) = (10, 5)
+    g = [np.zeros((N, N)) for _ in range(4)]
+    O.deposit_sort(p, a, *g)
+    w = np.array([1 / 384, 19 / 96, 115 / 192, 19 / 96, 1 / 384])
+    expect = np.zeros((N, N))
+    expect[8:13, 3:8] = np.outer(w, w) / h ** 2 * -0.0025
+    assert np.allclose(g[0], expect, rtol=1e-13, atol=1e-18)
+    assert not np.any(g[1]) and not np.any(g[2]) and not np.any(g[3])
+
+
+def test_deposit_conserves_charge_and_current():
+    N, h = 33, 0.1
+    p = O.Params(N=N, h=h, xi_step=0.05, subtraction_coeff=1.0)
+    rng = np.random.default_rng(4)
+    n = 500
+    a = dict(x_init=rng.uniform(-1, 1, n), y_init=rng.uniform(-1, 1, n), x_offt=np.zeros(n),
+             y_offt=np.zeros(n), px=rng.normal(0, .3, n), py=rng.normal(0, .3, n),
+             pz=rng.normal(0, .3, n), q=np.full(n, -0.01), m=np.full(n, 0.01))
+    g = [np.zeros((N, N)) for _ in range(4)]
+    O.deposit_sort(p, a, *g)
+    gamma = np.sqrt(a['m'] ** 2 + a['px'] ** 2 + a['py'] ** 2 + a['pz'] ** 2)
+    dro = a['q'] / (1 - a['pz'] / gamma)
+    assert g[0].sum() * h * h == pytest.approx(dro.sum(), rel=1e-12)
+    assert g[1].sum() * h * h == pytest.approx((a['px'] * dro / gamma).sum(), rel=1e-11)
+    assert g[3].sum() * h * h == pytest.approx((a['pz'] * dro / gamma).sum(), rel=1e-11)
+
+
+def test_banded_parallel_deposit_equals_serial_bitwise():
+    N, h = 65, 0.05
+    p = O.Params(N=N, h=h, xi_step=0.05, subtraction_coeff=1.0)
+    rng = np.random.default_rng(5)
+    n = 20000
+    a = dict(x_init=rng.uniform(-1.3, 1.3, n), y_init=rng.uniform(-1.3, 1.3, n),
+             x_offt=np.zeros(n), y_offt=np.zeros(n), px=rng.normal(0, .1, n),
+             py=rng.normal(0, .1, n), pz=rng.normal(0, .1, n), q=np.full(n, -1e-3),
+             m=np.full(n, 1e-3))
+    a['q'][::7] = 0
+    g1 = [np.zeros((N, N)) for _ in range(4)]
+    g8 = [np.zeros((N, N)) for _ in range(4)]
+    O.deposit_sort(p, a, *g1, nthreads=1)
+    O.deposit_sort(p, a, *g8, nthreads=8)
+    for u, v in zip(g1, g8):
+        assert np.array_equal(u, v)
+
+
+# -- plasma push (move.py:28-165) -----------------------------------------------
+def _push(p, fields, a, full=None):
+    prev = {'electrons': a}
+    full = O.move_wo_fields(p, prev) if full is None else full
+    return O.move_smart(p, fields, prev, full)['electrons']
+
+
+def test_free_stream_estimate():
+    p = O.Params(N=33, h=0.1, xi_step=0.05, subtraction_coeff=1.0)
+    a = _one_particle(0.0, 0.0, px=0.3, py=-0.2, pz=0.1)
+    g = math.sqrt(0.0025 ** 2 + 0.1 ** 2 + 0.3 ** 2 + 0.2 ** 2)
+    full = O.move_wo_fields(p, {'electrons': a})['electrons']
+    assert full['x_offt'][0, 0] == pytest.approx(0.3 / (g - 0.1) * 0.05, rel=1e-15)
+    assert full['y_offt'][0, 0] == pytest.approx(-0.2 / (g - 0.1) * 0.05, rel=1e-15)
+    assert a['x_offt'][0, 0] == 0.0     # the previous state is not touched
+
+
+def test_push_without_fields_is_free_streaming():
+    p = O.Params(N=33, h=0.1, xi_step=0.05, subtraction_coeff=1.0)
+    a = _one_particle(0.1, 0.2, px=0.3, py=-0.2, pz=0.1)
+    out = _push(p, O.zero_fields(33), a)
+    g = math.sqrt(0.0025 ** 2 + 0.1 ** 2 + 0.3 ** 2 + 0.2 ** 2)
+    assert out['px'][0, 0] == 0.3 and out['pz'][0, 0] == 0.1
+    assert out['x_offt'][0, 0] == pytest.approx(0.3 / (g - 0.1) * 0.05, rel=1e-14)
+
+
+def test_push_in_uniform_ez():
+    p = O.Params(N=33, h=0.1, xi_step=0.05, subtraction_coeff=1.0)
+    f = O.zero_fields(33)
+    f['Ez'][:] = 0.7
+    q, m = -0.0025, 0.0025
+    out = _push(p, f, _one_particle(0.0, 0.0, q=q, m=m))
+    p1 = q * 0.05 * 0.7 / 2
+    g1 = math.sqrt(m * m + p1 * p1)
+    assert out['pz'][0, 0] == pytest.approx(q * 0.05 / (1 - p1 / g1) * 0.7, rel=1e-14)
+    assert out['px'][0, 0] == 0 and out['x_offt'][0, 0] == 0
+
+
+def test_push_reflects_and_absorbs_at_the_wall():
+    p = O.Params(N=33, h=0.1, xi_step=0.05, subtraction_coeff=1.0)
+    b = p.boundary                               # 0.1 * (16.5 - 10) = 0.65
+    a = _one_particle(b - 0.01, 0.0, px=5.0, pz=0.0)
+    out = _push(p, O.zero_fields(33), a)
+    x = b - 0.01 + out['x_offt'][0, 0]
+    assert x < b and out['px'][0, 0] == 0 and out['q'][0, 0] != 0
+    a = _one_particle(5.5 * b, 0.0)              # still outside after both reflections
+    out = _push(p, O.zero_fields(33), a, full={'electrons': {k: v.copy() for k, v in a.items()}})
+    assert out['q'][0, 0] == 0 and out['x_offt'][0, 0] == 0
+
+
+def test_push_skips_absorbed_particles():
+    p = O.Params(N=33, h=0.1, xi_step=0.05, subtraction_coeff=1.0)
+    a = _one_particle(0.1, 0.1, px=0.2, q=0.0)
+    f = O.zero_fields(33)
+    f['Ex'][:] = 1.0
+    full = {'electrons': {k: v.copy() for k, v in a.items()}}
+    out = O.move_smart(p, f, {'electrons': a}, full)['electrons']
+    assert out['px'][0, 0] == 0.2 and out['x_offt'][0, 0] == 0
+
+
+# -- beam (beam3d) ------------------------------------------------------------------
+def _beam(n, rng, xi0, xi_step, r=0.5):
+    b = O.empty_beam()
+    b = {k: v for k, v in b.items()}
+    b['xi'] = rng.uniform(xi0 - xi_step, xi0, n)
+    b['x'] = rng.uniform(-r, r, n)
+    b['y'] = rng.uniform(-r, r, n)
+    b['px'] = rng.normal(0, 0.01, n)
+    b['py'] = rng.normal(0, 0.01, n)
+    b['pz'] = np.full(n, 1000.0)
+    b['q_m'] = np.full(n, -1.0)
+    b['q_norm'] = np.full(n, 0.01)
+    b['id'] = np.arange(1, n + 1, dtype=np.int64)
+    b['dt'] = np.zeros(n)
+    b['remaining_steps'] = np.zeros(n, dtype=np.int64)
+    return b
+
+
+def test_beam_deposit_two_layer_split_conserves_charge():
+    p = O.Params(N=33, h=0.1, xi_step=0.05, subtraction_coeff=1.0)
+    rng = np.random.default_rng(6)
+    calc = O.BeamCalc(p)
+    b = _beam(300, rng, -0.05 * 4, 0.05)
+    rho_now = calc.layout_beam_layer(b, 4)
+    total = (rho_now.sum() + calc.rho_layout.sum() / p.h ** 2) * p.h ** 2
+    assert total == pytest.approx(b['q_norm'].sum(), rel=1e-12)
+    xi_loc = (-0.05 * 4 - b['xi']) / 0.05
+    assert rho_now.sum() * p.h ** 2 == pytest.approx((b['q_norm'] * xi_loc).sum(), rel=1e-12)
+
+
+def test_beam_push_zero_field_drift_and_flags():
+    p = O.Params(N=33, h=0.1, xi_step=0.05, subtraction_coeff=1.0, time_step=0.5)
+    rng = np.random.default_rng(7)
+    b = _beam(4, rng, -0.05 * 3, 0.05)
+    b['xi'][:] = -0.17                            # inside layer [-0.2, -0.15)
+    b['x'][1] = 0.999 * p.lost_radius             # will cross the wall
+    b['px'][1] = 50.0
+    b['pz'][2] = 1.0                              # slow: falls to the next layer
+    z = O.zero_fields(33)
+    x0, px0 = b['x'].copy(), b['px'].copy()
+    lost, moved, fell = O.BeamCalc(p).move_beam_layer(b, 0, 4, z, z)
+    assert list(lost['id']) == [-2]
+    assert 3 in list(fell['id'])
+    assert 1 in list(moved['id'])
+    i = list(moved['id']).index(1)
+    g = math.sqrt(1 + px0[0] ** 2 + b['py'][0] ** 2 + 1000.0 ** 2)
+    assert moved['x'][i] == pytest.approx(x0[0] + 0.5 * px0[0] / g, rel=1e-12)
+    assert moved['px'][i] == px0[0]
+
+
+def test_beam_substep_multiplier():
+    qm = np.array([-1.0, -1.0, -1.0])
+    pz = np.array([1000.0, 1.0, 0.1])
+    dt = np.zeros(3)
+    O.lib().oracle_beam_substep(O.ctypes.c_int64(3), O._p(qm), O._p(pz), O.ctypes.c_double(2.0),
+                                O._p(dt))
+    for d, z in zip(dt, pz):
+        max_dt = math.sqrt(math.sqrt(1 + z * z) / 2.0)
+        assert d <= max_dt and (d == 1.0 or 2 * d > max_dt)
