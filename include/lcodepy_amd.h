@@ -1,0 +1,182 @@
+/*
+ * lcodepy_amd -- C ABI of the MI355X (gfx950) quasistatic 3D xi step.
+ *
+ * Each entry point stands in for one factory closure of lcodePy's
+ * plasma3d / beam3d packages, so the Python host layer can swap the
+ * reference's Numba / CuPy kernels for these one call site at a time:
+ *
+ *   lc3d_move_wo_fields   plasma3d/move.py:331-349  (move_particles_wo_fields)
+ *   lc3d_move_smart       plasma3d/move.py:351-381  (move_particles_smart)
+ *   lc3d_deposit          plasma3d/weights.py:249-285 (deposit_plasma) for one
+ *                         sort, with rhoj.py:11-20's background-ion column
+ *   lc3d_compute_fields   plasma3d/fields.py:276-322 (compute_fields)
+ *   lc3d_plasma_step      plasma3d/solver.py:25-78   (Plane2d3vPlasmaSolver.step_dxi)
+ *   lc3d_deposit_beam     beam3d/weights.py:152-163  (deposit_beam)
+ *   lc3d_beam_substep     beam3d/beam_calculator.py:97-114 (start_moving_layer)
+ *   lc3d_move_beam        beam3d/move.py:294-309     (move_beam_particles)
+ *
+ * Conventions
+ *   - All pointers are device pointers (hipMalloc'd or torch-ROCm tensors),
+ *     C-contiguous.  (N, N) grids are indexed [ix, iy] -> ix * N + iy;
+ *     current grids are (2, N, N) with column = sort (0 electrons, 1 ions),
+ *     as plasma3d/initialization.py:391 lays them out.
+ *   - Precision is fixed per context:
+ *       LC3D_FP64       every array float64 (the reference's precision);
+ *       LC3D_FP32_BULK  x_init, y_init, x_offt, y_offt and the fields of
+ *                       layers k / k+1 (fields_prev, fields_full) float64;
+ *                       px, py, pz, q, m, currents and the half-step fields
+ *                       float32.
+ *     Beam arrays are float64 (ids and step counters int64, masks uint8) in
+ *     both forms.
+ *   - `stream` is a hipStream_t (NULL = default stream).  Work is
+ *     stream-ordered; no call synchronises the host except
+ *     lc3d_error_flags.
+ *   - Return value: 0 on success, otherwise an LC3D_E* code; the message is
+ *     available from lc3d_last_error(ctx).  The library never frees a
+ *     caller's buffer and allocates only inside lc3d_create.
+ *   - One context per host thread; calls on one context are not reentrant.
+ */
+#ifndef LCODEPY_AMD_H
+#define LCODEPY_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define LC3D_FP64 0
+#define LC3D_FP32_BULK 1
+
+#define LC3D_OK 0
+#define LC3D_EINVAL 1    /* bad argument (null pointer, size beyond the context) */
+#define LC3D_EHIP 2      /* a HIP runtime call or kernel launch failed */
+#define LC3D_ENOMEM 3    /* device allocation in lc3d_create failed */
+
+/* Sticky device error bits (lc3d_error_flags). */
+#define LC3D_ERR_PARTICLE_OFF_GRID 1u  /* plasma stencil left the grid: not deposited */
+#define LC3D_ERR_NONFINITE 2u          /* non-finite plasma position: not deposited */
+#define LC3D_ERR_BEAM_OFF_GRID 4u      /* beam stencil partly off the grid / substep cap hit */
+#define LC3D_ERR_CELL_OVERFLOW 8u      /* > 4096 particles in one cell (summed unsorted) */
+
+typedef struct lc3d_ctx lc3d_ctx;
+
+/* Geometry and numerics of one run; the values of the reference's config
+ * keys after its 3D adjustment (config/config.py:185-234). */
+typedef struct lc3d_params {
+    int32_t grid_steps;          /* window-width-steps N: odd, N - 1 <= 1024 */
+    double grid_step_size;       /* transverse-step h */
+    double xi_step_size;         /* xi-step */
+    double boundary;             /* h (N/2 - bound-padding-steps), move.py:321 */
+    double subtraction_coeff;    /* field-solver-subtraction-coefficient */
+    double lost_radius;          /* max(0.9 R, R - 1), R = h N / 2, beam3d/move.py:285 */
+    int32_t precision;           /* LC3D_FP64 or LC3D_FP32_BULK */
+    int64_t max_particles;       /* largest plasma sort the context will deposit */
+    /* radix plan of the length-(N-1) complex FFT; 0 stages = library default */
+    int32_t fft_nstages;
+    int32_t fft_radix[12];
+    /* host array of the N eigenvalues 4/h^2 sin^2(k pi / (2(N-1))) used by
+     * the solver matrices (initialization.py:24, 43-44, 63); NULL = computed
+     * here with the C library's sin */
+    const double *eigenvalues;
+} lc3d_params;
+
+/* Allocates every workspace (cell bins, spectral scratch, twiddles,
+ * eigenvalues) on the current HIP device. */
+int lc3d_create(const lc3d_params *params, lc3d_ctx **out);
+void lc3d_destroy(lc3d_ctx *ctx);
+const char *lc3d_last_error(const lc3d_ctx *ctx);
+int lc3d_precision(const lc3d_ctx *ctx);
+
+/* Copies the sticky device error bits to *flags and clears them
+ * (synchronises `stream`). */
+int lc3d_error_flags(lc3d_ctx *ctx, uint32_t *flags, void *stream);
+
+/* ---- plasma particles ------------------------------------------------------ */
+typedef struct lc3d_fields6 {   /* the six fields a push reads */
+    const void *Ex, *Ey, *Ez, *Bx, *By, *Bz;
+} lc3d_fields6;
+
+typedef struct lc3d_sort {
+    int64_t n;                         /* macro-particles of this sort */
+    const double *x_init, *y_init;     /* constant lattice positions */
+    const void *m;                     /* mass, constant */
+    const double *x_offt_prev, *y_offt_prev;   /* layer k */
+    const void *px_prev, *py_prev, *pz_prev, *q_prev;
+    double *x_offt, *y_offt;                   /* layer k+1 (written) */
+    void *px, *py, *pz, *q;
+    int32_t column;                    /* current column of this sort */
+} lc3d_sort;
+
+/* x_offt += px / (gamma - pz) * xi_step where m != 0 (in place). */
+int lc3d_move_wo_fields(lc3d_ctx *ctx, int64_t n, const void *m, double *x_offt, double *y_offt,
+                        const void *px, const void *py, const void *pz, void *stream);
+
+/* Predictor-corrector push of one sort: reads the *_prev arrays and the
+ * current estimate in x_offt / y_offt / q, writes x_offt, y_offt, px, py, pz
+ * and q (absorbed particles get q = 0).  field_fp32 != 0 when the fields
+ * are float32 (the half-step fields of the FP32-bulk form). */
+int lc3d_move_smart(lc3d_ctx *ctx, const lc3d_sort *sort, const lc3d_fields6 *fields,
+                    int32_t field_fp32, void *stream);
+
+/* Deposits one sort's layer-k+1 state (x_offt, ..., q) into the (N, N)
+ * grids ro, jx, jy, jz (overwritten).  With bg_ro != NULL also writes the
+ * background-ion column: ro1 = bg_ro, jx1 = jy1 = jz1 = 0. */
+int lc3d_deposit(lc3d_ctx *ctx, const lc3d_sort *sort, void *ro, void *jx, void *jy, void *jz,
+                 const void *bg_ro, void *ro1, void *jx1, void *jy1, void *jz1, void *stream);
+
+/* ---- fields ------------------------------------------------------------------- */
+typedef struct lc3d_fields {   /* Ex Ey Ez Bx By Bz Phi, each (N, N) */
+    void *Ex, *Ey, *Ez, *Bx, *By, *Bz, *Phi;
+} lc3d_fields;
+
+typedef struct lc3d_currents { /* ro jx jy jz, each (2, N, N) */
+    void *ro, *jx, *jy, *jz;
+} lc3d_currents;
+
+/* compute_fields(fields, fields_prev, const, rho_beam_full, rho_beam_prev,
+ * currents_prev, currents_full) -> (fields_full, fields_half).  `pred` is
+ * the Helmholtz predictor (only Ex Ey Bx By are read); pred_fp32 != 0 when
+ * it is float32. */
+int lc3d_compute_fields(lc3d_ctx *ctx, const lc3d_fields *pred, int32_t pred_fp32,
+                        const lc3d_fields *fields_prev, const double *rho_beam_full,
+                        const double *rho_beam_prev, const lc3d_currents *currents_prev,
+                        const lc3d_currents *currents_full, lc3d_fields *fields_full,
+                        lc3d_fields *fields_half, void *stream);
+
+/* One whole xi step of the plasma: free stream, 3 pushes, 3 deposits and 2
+ * field solves in the reference's order.  Writes every sort's layer-k+1
+ * arrays, fields_full, fields_half and currents_full (column 1 holds
+ * bg_ro when there is one sort and bg_ro != NULL). */
+int lc3d_plasma_step(lc3d_ctx *ctx, int32_t n_sorts, const lc3d_sort *sorts,
+                     const lc3d_fields *fields_prev, const lc3d_currents *currents_prev,
+                     const double *rho_beam_full, const double *rho_beam_prev, const void *bg_ro,
+                     lc3d_fields *fields_full, lc3d_fields *fields_half,
+                     const lc3d_currents *currents_full, void *stream);
+/* The beam charge densities rho_beam_* are float64 (N, N) in both forms. */
+
+/* ---- beam ----------------------------------------------------------------------- */
+/* Adds q_norm * w * xi_loc into ro_running and q_norm * w * (1 - xi_loc)
+ * into ro_next for the n particles of a layer (both accumulate). */
+int lc3d_deposit_beam(lc3d_ctx *ctx, int64_t n, int64_t plasma_layer_idx, const double *x,
+                      const double *y, const double *xi, const double *q_norm,
+                      double *ro_running, double *ro_next, void *stream);
+
+/* dt = mult * time_step and remaining_steps = int(1 / mult) for n particles. */
+int lc3d_beam_substep(lc3d_ctx *ctx, int64_t n, const double *q_m, const double *pz,
+                      double substepping_energy, double time_step, double *dt,
+                      int64_t *remaining_steps, void *stream);
+
+/* Moves a layer through fields_k (previous plasma layer) and fields_k1 (the
+ * one just computed); fields are float64.  lost/moved/fell are written. */
+int lc3d_move_beam(lc3d_ctx *ctx, int64_t n, int64_t beam_layer_idx, const double *q_m,
+                   const double *dt, const lc3d_fields6 *fields_k, const lc3d_fields6 *fields_k1,
+                   int64_t *remaining_steps, int64_t *id, double *x, double *y, double *xi,
+                   double *px, double *py, double *pz, uint8_t *lost, uint8_t *moved,
+                   uint8_t *fell, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* LCODEPY_AMD_H */

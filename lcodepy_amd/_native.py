@@ -1,0 +1,196 @@
+"""ctypes binding of ``liblcodepy_amd.so`` (the C ABI in include/lcodepy_amd.h).
+
+The library is built in-tree by ``__graft_entry__.build()`` (or
+``make -C lcodepy_amd/csrc``).  There is no fallback: if the library or a GPU
+is missing, :func:`lib` raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import threading
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, 'liblcodepy_amd.so')
+
+LC3D_FP64 = 0
+LC3D_FP32_BULK = 1
+
+ERR_BITS = {
+    1: 'plasma particle stencil left the grid (not deposited)',
+    2: 'non-finite plasma particle position (not deposited)',
+    4: 'beam particle stencil partly off the grid or substep cap reached',
+    8: 'more than 4096 plasma particles in one cell (summed unsorted)',
+}
+
+_vp = ctypes.c_void_p
+_dp = ctypes.c_void_p   # device pointers are passed as integers
+
+
+class Params(ctypes.Structure):
+    _fields_ = [('grid_steps', ctypes.c_int32),
+                ('grid_step_size', ctypes.c_double),
+                ('xi_step_size', ctypes.c_double),
+                ('boundary', ctypes.c_double),
+                ('subtraction_coeff', ctypes.c_double),
+                ('lost_radius', ctypes.c_double),
+                ('precision', ctypes.c_int32),
+                ('max_particles', ctypes.c_int64),
+                ('fft_nstages', ctypes.c_int32),
+                ('fft_radix', ctypes.c_int32 * 12),
+                ('eigenvalues', ctypes.POINTER(ctypes.c_double))]
+
+
+class Fields6(ctypes.Structure):
+    _fields_ = [(k, _vp) for k in ('Ex', 'Ey', 'Ez', 'Bx', 'By', 'Bz')]
+
+
+class Fields(ctypes.Structure):
+    _fields_ = [(k, _vp) for k in ('Ex', 'Ey', 'Ez', 'Bx', 'By', 'Bz', 'Phi')]
+
+
+class Currents(ctypes.Structure):
+    _fields_ = [(k, _vp) for k in ('ro', 'jx', 'jy', 'jz')]
+
+
+class Sort(ctypes.Structure):
+    _fields_ = [('n', ctypes.c_int64),
+                ('x_init', _vp), ('y_init', _vp), ('m', _vp),
+                ('x_offt_prev', _vp), ('y_offt_prev', _vp),
+                ('px_prev', _vp), ('py_prev', _vp), ('pz_prev', _vp), ('q_prev', _vp),
+                ('x_offt', _vp), ('y_offt', _vp),
+                ('px', _vp), ('py', _vp), ('pz', _vp), ('q', _vp),
+                ('column', ctypes.c_int32)]
+
+
+SYMBOLS = ('lc3d_create', 'lc3d_destroy', 'lc3d_last_error', 'lc3d_precision',
+           'lc3d_error_flags', 'lc3d_move_wo_fields', 'lc3d_move_smart', 'lc3d_deposit',
+           'lc3d_compute_fields', 'lc3d_plasma_step', 'lc3d_deposit_beam',
+           'lc3d_beam_substep', 'lc3d_move_beam')
+
+_LIB = None
+_LOCK = threading.Lock()
+
+
+def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
+    """dlopen the library and declare the signatures (no GPU needed)."""
+    if not os.path.exists(path):
+        raise RuntimeError(f'{path} is missing: build it with __graft_entry__.build() '
+                           'or make -C lcodepy_amd/csrc (there is no CPU fallback)')
+    L = ctypes.CDLL(path)
+    i32, i64, dbl = ctypes.c_int32, ctypes.c_int64, ctypes.c_double
+    P = ctypes.POINTER
+    L.lc3d_create.argtypes = [P(Params), P(_vp)]
+    L.lc3d_create.restype = ctypes.c_int
+    L.lc3d_destroy.argtypes = [_vp]
+    L.lc3d_destroy.restype = None
+    L.lc3d_last_error.argtypes = [_vp]
+    L.lc3d_last_error.restype = ctypes.c_char_p
+    L.lc3d_precision.argtypes = [_vp]
+    L.lc3d_error_flags.argtypes = [_vp, P(ctypes.c_uint32), _vp]
+    L.lc3d_move_wo_fields.argtypes = [_vp, i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp]
+    L.lc3d_move_smart.argtypes = [_vp, P(Sort), P(Fields6), i32, _vp]
+    L.lc3d_deposit.argtypes = [_vp, P(Sort), _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]
+    L.lc3d_compute_fields.argtypes = [_vp, P(Fields), i32, P(Fields), _vp, _vp, P(Currents),
+                                      P(Currents), P(Fields), P(Fields), _vp]
+    L.lc3d_plasma_step.argtypes = [_vp, i32, P(Sort), P(Fields), P(Currents), _vp, _vp, _vp,
+                                   P(Fields), P(Fields), P(Currents), _vp]
+    L.lc3d_deposit_beam.argtypes = [_vp, i64, i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp]
+    L.lc3d_beam_substep.argtypes = [_vp, i64, _vp, _vp, dbl, dbl, _vp, _vp, _vp]
+    L.lc3d_move_beam.argtypes = [_vp, i64, i64, _vp, _vp, P(Fields6), P(Fields6), _vp, _vp,
+                                 _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]
+    for name in SYMBOLS:
+        if name not in ('lc3d_destroy', 'lc3d_last_error'):
+            getattr(L, name).restype = ctypes.c_int
+    return L
+
+
+def lib() -> ctypes.CDLL:
+    global _LIB
+    with _LOCK:
+        if _LIB is None:
+            _LIB = load_library()
+    return _LIB
+
+
+def ptr(t) -> int | None:
+    """Device address of a torch tensor (None for None)."""
+    if t is None:
+        return None
+    if not t.is_cuda:
+        raise ValueError('lcodepy_amd kernels need device tensors')
+    if not t.is_contiguous():
+        raise ValueError('lcodepy_amd kernels need C-contiguous tensors')
+    return t.data_ptr()
+
+
+def stream_ptr(stream=None) -> int:
+    import torch
+    s = torch.cuda.current_stream() if stream is None else stream
+    return s.cuda_stream
+
+
+class Context:
+    """One lc3d context: geometry, precision and workspaces on one device."""
+
+    def __init__(self, N: int, h: float, xi_step: float, boundary: float,
+                 subtraction_coeff: float, lost_radius: float, precision: str,
+                 max_particles: int, eigenvalues: np.ndarray | None = None,
+                 fft_radix: tuple | None = None):
+        import torch
+        if not torch.cuda.is_available():
+            raise RuntimeError('lcodepy_amd needs a ROCm GPU (torch.cuda.is_available() is False)')
+        self.lib = lib()
+        self.N = N
+        self.precision = precision
+        self.max_particles = int(max_particles)
+        p = Params()
+        p.grid_steps = N
+        p.grid_step_size = h
+        p.xi_step_size = xi_step
+        p.boundary = boundary
+        p.subtraction_coeff = subtraction_coeff
+        p.lost_radius = lost_radius
+        p.precision = LC3D_FP32_BULK if precision == 'fp32-bulk' else LC3D_FP64
+        p.max_particles = self.max_particles
+        if fft_radix:
+            p.fft_nstages = len(fft_radix)
+            for i, r in enumerate(fft_radix):
+                p.fft_radix[i] = r
+        self._lam = None
+        if eigenvalues is not None:
+            self._lam = np.ascontiguousarray(eigenvalues, dtype=np.float64)
+            p.eigenvalues = self._lam.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+        h_ = _vp()
+        rc = self.lib.lc3d_create(ctypes.byref(p), ctypes.byref(h_))
+        if rc != 0:
+            raise RuntimeError(f'lc3d_create failed with code {rc} (N={N}, precision={precision})')
+        self.handle = h_
+
+    def __del__(self):
+        h = getattr(self, 'handle', None)
+        if h:
+            try:
+                self.lib.lc3d_destroy(h)
+            except Exception:
+                pass
+            self.handle = None
+
+    def check(self, rc: int, what: str) -> None:
+        if rc != 0:
+            msg = self.lib.lc3d_last_error(self.handle)
+            raise RuntimeError(f'{what} failed ({rc}): {msg.decode() if msg else ""}')
+
+    def error_flags(self, stream=None) -> int:
+        v = ctypes.c_uint32(0)
+        self.check(self.lib.lc3d_error_flags(self.handle, ctypes.byref(v), stream_ptr(stream)),
+                   'lc3d_error_flags')
+        return int(v.value)
+
+    def raise_on_errors(self, stream=None) -> None:
+        flags = self.error_flags(stream)
+        if flags:
+            msgs = [m for b, m in ERR_BITS.items() if flags & b]
+            raise FloatingPointError('lcodepy_amd kernels reported: ' + '; '.join(msgs))

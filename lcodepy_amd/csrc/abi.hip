@@ -1,0 +1,545 @@
+// C ABI of lcodepy_amd (declared in include/lcodepy_amd.h): context set-up
+// and the entry points that dispatch the gfx950 kernels by precision.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/lcodepy_amd.h"
+#include "kernels.h"
+
+using namespace lc3d;
+
+struct lc3d_ctx {
+    lc3d_params prm{};
+    int N = 0;
+    int device = 0;
+    int64_t cap = 0;
+    FftPlan plan{};
+    // binning
+    int *count = nullptr, *start = nullptr, *blocks = nullptr, *key = nullptr, *idx = nullptr;
+    unsigned *err = nullptr;
+    // field solve
+    void *work = nullptr;
+    double *rowsum = nullptr;
+    void *tw = nullptr;
+    double *lam = nullptr;
+    std::string last_error;
+};
+
+namespace {
+
+int fail(lc3d_ctx *c, int code, const std::string &msg)
+{
+    if (c) c->last_error = msg;
+    return code;
+}
+
+int hip_fail(lc3d_ctx *c, hipError_t e, const char *where)
+{
+    return fail(c, LC3D_EHIP, std::string(where) + ": " + hipGetErrorString(e));
+}
+
+#define HIP_TRY(ctx, expr)                                   \
+    do {                                                     \
+        hipError_t e_ = (expr);                              \
+        if (e_ != hipSuccess) return hip_fail(ctx, e_, #expr); \
+    } while (0)
+
+bool default_plan(int M, FftPlan &p)
+{
+    p.M = M;
+    p.ns = 0;
+    int m = M, twos = 0;
+    while (m % 2 == 0) {
+        m /= 2;
+        ++twos;
+    }
+    while (twos >= 3) {
+        p.radix[p.ns++] = 8;
+        twos -= 3;
+    }
+    if (twos == 2) p.radix[p.ns++] = 4;
+    if (twos == 1) p.radix[p.ns++] = 2;
+    const int odd[] = {3, 5, 7, 11, 13};
+    for (int r : odd) {
+        while (m % r == 0) {
+            if (p.ns >= 12) return false;
+            p.radix[p.ns++] = r;
+            m /= r;
+        }
+    }
+    return m == 1 && p.ns <= 12;
+}
+
+bool valid_plan(const FftPlan &p)
+{
+    int prod = 1;
+    for (int s = 0; s < p.ns; ++s) {
+        const int r = p.radix[s];
+        if (r != 2 && r != 3 && r != 4 && r != 5 && r != 7 && r != 8 && r != 11 && r != 13)
+            return false;
+        prod *= r;
+    }
+    return prod == p.M;
+}
+
+bool fp32(const lc3d_ctx *c) { return c->prm.precision == LC3D_FP32_BULK; }
+
+PushGeom push_geom(const lc3d_ctx *c, int64_t n)
+{
+    return PushGeom{n, c->N, c->prm.grid_step_size, c->prm.xi_step_size, c->prm.boundary};
+}
+
+BinState bin_state(const lc3d_ctx *c)
+{
+    return BinState{c->count, c->start, c->blocks, c->key, c->idx, c->err};
+}
+
+template <typename TP> SortArrays<TP> sort_arrays(const lc3d_sort *s)
+{
+    SortArrays<TP> a;
+    a.x_init = s->x_init;
+    a.y_init = s->y_init;
+    a.m = static_cast<const TP *>(s->m);
+    a.xo_prev = s->x_offt_prev;
+    a.yo_prev = s->y_offt_prev;
+    a.px_prev = static_cast<const TP *>(s->px_prev);
+    a.py_prev = static_cast<const TP *>(s->py_prev);
+    a.pz_prev = static_cast<const TP *>(s->pz_prev);
+    a.q_prev = static_cast<const TP *>(s->q_prev);
+    a.xo = s->x_offt;
+    a.yo = s->y_offt;
+    a.px = static_cast<TP *>(s->px);
+    a.py = static_cast<TP *>(s->py);
+    a.pz = static_cast<TP *>(s->pz);
+    a.q = static_cast<TP *>(s->q);
+    return a;
+}
+
+template <typename T> Fields6<T> f6(const lc3d_fields6 *f)
+{
+    return Fields6<T>{static_cast<const T *>(f->Ex), static_cast<const T *>(f->Ey),
+                      static_cast<const T *>(f->Ez), static_cast<const T *>(f->Bx),
+                      static_cast<const T *>(f->By), static_cast<const T *>(f->Bz)};
+}
+
+template <typename T> Fields6<T> f6(const lc3d_fields *f)
+{
+    return Fields6<T>{static_cast<const T *>(f->Ex), static_cast<const T *>(f->Ey),
+                      static_cast<const T *>(f->Ez), static_cast<const T *>(f->Bx),
+                      static_cast<const T *>(f->By), static_cast<const T *>(f->Bz)};
+}
+
+SolveGeom solve_geom(const lc3d_ctx *c)
+{
+    return SolveGeom{c->N, c->prm.grid_step_size, c->prm.xi_step_size, c->prm.subtraction_coeff,
+                     c->plan, c->lam, c->tw, c->work, c->rowsum};
+}
+
+template <typename TG, typename TPRED>
+SolveIn<TG, TPRED> solve_in(const lc3d_ctx *c, const lc3d_fields *pred, const double *rbF,
+                            const double *rbP, const lc3d_currents *cp, const lc3d_currents *cf)
+{
+    const size_t nn = (size_t)c->N * c->N;
+    SolveIn<TG, TPRED> in;
+    auto col = [&](void *p, int k) { return static_cast<const TG *>(p) + k * nn; };
+    for (int k = 0; k < 2; ++k) {
+        in.roF[k] = col(cf->ro, k);
+        in.jxF[k] = col(cf->jx, k);
+        in.jyF[k] = col(cf->jy, k);
+        in.jzF[k] = col(cf->jz, k);
+        in.roP[k] = col(cp->ro, k);
+        in.jxP[k] = col(cp->jx, k);
+        in.jyP[k] = col(cp->jy, k);
+        in.jzP[k] = col(cp->jz, k);
+    }
+    in.rbF = rbF;
+    in.rbP = rbP;
+    in.ExP = static_cast<const TPRED *>(pred->Ex);
+    in.EyP = static_cast<const TPRED *>(pred->Ey);
+    in.BxP = static_cast<const TPRED *>(pred->Bx);
+    in.ByP = static_cast<const TPRED *>(pred->By);
+    return in;
+}
+
+// Field order of the solve: Ex Ey Bx By Ez Bz Phi.
+template <typename TFULL, typename THALF>
+SolveOut<TFULL, THALF> solve_out(const lc3d_fields *prev, lc3d_fields *full, lc3d_fields *half)
+{
+    SolveOut<TFULL, THALF> o;
+    void *const p[7] = {prev->Ex, prev->Ey, prev->Bx, prev->By, prev->Ez, prev->Bz, prev->Phi};
+    void *const f[7] = {full->Ex, full->Ey, full->Bx, full->By, full->Ez, full->Bz, full->Phi};
+    void *const h[7] = {half->Ex, half->Ey, half->Bx, half->By, half->Ez, half->Bz, half->Phi};
+    for (int i = 0; i < 7; ++i) {
+        o.prev[i] = static_cast<const TFULL *>(p[i]);
+        o.full[i] = static_cast<TFULL *>(f[i]);
+        o.half[i] = static_cast<THALF *>(h[i]);
+    }
+    return o;
+}
+
+bool sort_ok(const lc3d_ctx *c, const lc3d_sort *s)
+{
+    return s && s->n >= 0 && s->n <= c->cap && (s->n == 0 || (s->x_init && s->y_init && s->m &&
+           s->x_offt_prev && s->y_offt_prev && s->px_prev && s->py_prev && s->pz_prev &&
+           s->q_prev && s->x_offt && s->y_offt && s->px && s->py && s->pz && s->q)) &&
+           (s->column == 0 || s->column == 1);
+}
+
+bool fields_ok(const lc3d_fields *f, bool phi)
+{
+    return f && f->Ex && f->Ey && f->Ez && f->Bx && f->By && f->Bz && (!phi || f->Phi);
+}
+
+bool currents_ok(const lc3d_currents *c) { return c && c->ro && c->jx && c->jy && c->jz; }
+
+// Deposit of one sort into its column of `cur`.
+int deposit_sort(lc3d_ctx *c, const lc3d_sort *s, bool binned, const lc3d_currents *cur,
+                 const void *bg_ro, bool single_sort, hipStream_t st)
+{
+    const size_t nn = (size_t)c->N * c->N;
+    const int col = s->column;
+    const bool bg = bg_ro != nullptr && single_sort;
+    hipError_t e;
+    if (fp32(c)) {
+        auto at = [&](void *p, int k) { return static_cast<float *>(p) + k * nn; };
+        e = launch_deposit<float, float>(push_geom(c, s->n), sort_arrays<float>(s), true, binned,
+                                         bin_state(c), at(cur->ro, col), at(cur->jx, col),
+                                         at(cur->jy, col), at(cur->jz, col),
+                                         bg ? static_cast<const float *>(bg_ro) : nullptr,
+                                         at(cur->ro, 1), at(cur->jx, 1), at(cur->jy, 1),
+                                         at(cur->jz, 1), st);
+    } else {
+        auto at = [&](void *p, int k) { return static_cast<double *>(p) + k * nn; };
+        e = launch_deposit<double, double>(push_geom(c, s->n), sort_arrays<double>(s), true, binned,
+                                           bin_state(c), at(cur->ro, col), at(cur->jx, col),
+                                           at(cur->jy, col), at(cur->jz, col),
+                                           bg ? static_cast<const double *>(bg_ro) : nullptr,
+                                           at(cur->ro, 1), at(cur->jx, 1), at(cur->jy, 1),
+                                           at(cur->jz, 1), st);
+    }
+    if (e != hipSuccess) return hip_fail(c, e, "deposit");
+    return LC3D_OK;
+}
+
+int push_sort(lc3d_ctx *c, const lc3d_sort *s, const void *const *f, bool f32, bool first,
+              bool bin, hipStream_t st)
+{
+    const lc3d_fields6 fl{f[0], f[1], f[2], f[3], f[4], f[5]};
+    hipError_t e;
+    if (fp32(c)) {
+        if (f32)
+            e = launch_push<float, float>(push_geom(c, s->n), f6<float>(&fl), sort_arrays<float>(s),
+                                          first, bin, bin_state(c), st);
+        else
+            e = launch_push<float, double>(push_geom(c, s->n), f6<double>(&fl),
+                                           sort_arrays<float>(s), first, bin, bin_state(c), st);
+    } else {
+        if (f32) return fail(c, LC3D_EINVAL, "float32 fields need an FP32-bulk context");
+        e = launch_push<double, double>(push_geom(c, s->n), f6<double>(&fl),
+                                        sort_arrays<double>(s), first, bin, bin_state(c), st);
+    }
+    if (e != hipSuccess) return hip_fail(c, e, "push");
+    return LC3D_OK;
+}
+
+int solve(lc3d_ctx *c, const lc3d_fields *pred, bool pred_f32, const lc3d_fields *prev,
+          const double *rbF, const double *rbP, const lc3d_currents *cp, const lc3d_currents *cf,
+          lc3d_fields *full, lc3d_fields *half, bool write_full, hipStream_t st)
+{
+    hipError_t e;
+    const SolveGeom g = solve_geom(c);
+    if (fp32(c)) {
+        if (pred_f32)
+            e = launch_compute_fields<float, float, double, float>(
+                g, solve_in<float, float>(c, pred, rbF, rbP, cp, cf),
+                solve_out<double, float>(prev, full, half), write_full, st);
+        else
+            e = launch_compute_fields<float, double, double, float>(
+                g, solve_in<float, double>(c, pred, rbF, rbP, cp, cf),
+                solve_out<double, float>(prev, full, half), write_full, st);
+    } else {
+        if (pred_f32) return fail(c, LC3D_EINVAL, "float32 predictor needs an FP32-bulk context");
+        e = launch_compute_fields<double, double, double, double>(
+            g, solve_in<double, double>(c, pred, rbF, rbP, cp, cf),
+            solve_out<double, double>(prev, full, half), write_full, st);
+    }
+    if (e != hipSuccess) return hip_fail(c, e, "compute_fields");
+    return LC3D_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int lc3d_create(const lc3d_params *p, lc3d_ctx **out)
+{
+    if (!p || !out) return LC3D_EINVAL;
+    *out = nullptr;
+    const int N = p->grid_steps;
+    if (N < 9 || N % 2 == 0 || N - 1 > 1024 || p->grid_step_size <= 0 || p->xi_step_size <= 0 ||
+        p->max_particles < 0 || p->max_particles > 0x7fffffff ||
+        (p->precision != LC3D_FP64 && p->precision != LC3D_FP32_BULK))
+        return LC3D_EINVAL;
+    lc3d_ctx *c = new lc3d_ctx();
+    c->prm = *p;
+    c->prm.eigenvalues = nullptr;
+    c->N = N;
+    c->cap = p->max_particles;
+    const int M = N - 1;
+    if (p->fft_nstages > 0) {
+        c->plan.M = M;
+        c->plan.ns = p->fft_nstages;
+        for (int s = 0; s < 12; ++s) c->plan.radix[s] = s < p->fft_nstages ? p->fft_radix[s] : 0;
+        if (p->fft_nstages > 12 || !valid_plan(c->plan)) {
+            delete c;
+            return LC3D_EINVAL;
+        }
+    } else if (!default_plan(M, c->plan)) {
+        delete c;
+        return LC3D_EINVAL;
+    }
+    (void)hipGetDevice(&c->device);
+    const size_t nn = (size_t)N * N;
+    const size_t nblocks = (nn + 1 + 2047) / 2048;
+    const size_t tg = p->precision == LC3D_FP32_BULK ? sizeof(float) : sizeof(double);
+    const size_t cap = (size_t)std::max<int64_t>(1, c->cap);
+    bool ok = hipMalloc(&c->count, nn * sizeof(int)) == hipSuccess &&
+              hipMalloc(&c->start, (nn + 1) * sizeof(int)) == hipSuccess &&
+              hipMalloc(&c->blocks, nblocks * sizeof(int)) == hipSuccess &&
+              hipMalloc(&c->key, cap * sizeof(int)) == hipSuccess &&
+              hipMalloc(&c->idx, cap * sizeof(int)) == hipSuccess &&
+              hipMalloc(&c->err, sizeof(unsigned)) == hipSuccess &&
+              hipMalloc(&c->work, 7 * nn * tg) == hipSuccess &&
+              hipMalloc(&c->rowsum, N * sizeof(double)) == hipSuccess &&
+              hipMalloc(&c->tw, 2 * (size_t)M * 2 * tg) == hipSuccess &&
+              hipMalloc(&c->lam, N * sizeof(double)) == hipSuccess;
+    if (!ok) {
+        lc3d_destroy(c);
+        return LC3D_ENOMEM;
+    }
+    // Twiddles exp(-i pi t / M), t = 0..2M-1, and the eigenvalues, in float64.
+    std::vector<double> twd(4 * (size_t)M), lam(N);
+    const double pi = 3.14159265358979323846;
+    for (int t = 0; t < 2 * M; ++t) {
+        twd[2 * t] = std::cos(pi * t / M);
+        twd[2 * t + 1] = -std::sin(pi * t / M);
+    }
+    for (int k = 0; k < N; ++k) {
+        if (p->eigenvalues) {
+            lam[k] = p->eigenvalues[k];
+        } else {
+            const double s = std::sin(k * pi / (2.0 * (N - 1)));
+            lam[k] = 4 / (p->grid_step_size * p->grid_step_size) * (s * s);
+        }
+    }
+    hipError_t e = hipSuccess;
+    if (p->precision == LC3D_FP32_BULK) {
+        std::vector<float> twf(twd.begin(), twd.end());
+        e = hipMemcpy(c->tw, twf.data(), twf.size() * sizeof(float), hipMemcpyHostToDevice);
+    } else {
+        e = hipMemcpy(c->tw, twd.data(), twd.size() * sizeof(double), hipMemcpyHostToDevice);
+    }
+    if (e == hipSuccess) e = hipMemcpy(c->lam, lam.data(), N * sizeof(double), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemset(c->count, 0, nn * sizeof(int));
+    if (e == hipSuccess) e = hipMemset(c->err, 0, sizeof(unsigned));
+    if (e == hipSuccess) e = hipDeviceSynchronize();
+    if (e != hipSuccess) {
+        lc3d_destroy(c);
+        return LC3D_EHIP;
+    }
+    *out = c;
+    return LC3D_OK;
+}
+
+void lc3d_destroy(lc3d_ctx *c)
+{
+    if (!c) return;
+    (void)hipFree(c->count);
+    (void)hipFree(c->start);
+    (void)hipFree(c->blocks);
+    (void)hipFree(c->key);
+    (void)hipFree(c->idx);
+    (void)hipFree(c->err);
+    (void)hipFree(c->work);
+    (void)hipFree(c->rowsum);
+    (void)hipFree(c->tw);
+    (void)hipFree(c->lam);
+    delete c;
+}
+
+const char *lc3d_last_error(const lc3d_ctx *c) { return c ? c->last_error.c_str() : "null context"; }
+
+int lc3d_precision(const lc3d_ctx *c) { return c ? c->prm.precision : -1; }
+
+int lc3d_error_flags(lc3d_ctx *c, uint32_t *flags, void *stream)
+{
+    if (!c || !flags) return LC3D_EINVAL;
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    unsigned v = 0;
+    HIP_TRY(c, hipMemcpyAsync(&v, c->err, sizeof(unsigned), hipMemcpyDeviceToHost, st));
+    HIP_TRY(c, hipStreamSynchronize(st));
+    HIP_TRY(c, hipMemsetAsync(c->err, 0, sizeof(unsigned), st));
+    HIP_TRY(c, hipStreamSynchronize(st));
+    *flags = v;
+    return LC3D_OK;
+}
+
+int lc3d_move_wo_fields(lc3d_ctx *c, int64_t n, const void *m, double *x_offt, double *y_offt,
+                        const void *px, const void *py, const void *pz, void *stream)
+{
+    if (!c || n < 0 || (n > 0 && (!m || !x_offt || !y_offt || !px || !py || !pz)))
+        return fail(c, LC3D_EINVAL, "move_wo_fields: bad arguments");
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    hipError_t e;
+    if (fp32(c))
+        e = launch_move_wo_fields<float>(n, c->prm.xi_step_size, static_cast<const float *>(m),
+                                         x_offt, y_offt, static_cast<const float *>(px),
+                                         static_cast<const float *>(py),
+                                         static_cast<const float *>(pz), st);
+    else
+        e = launch_move_wo_fields<double>(n, c->prm.xi_step_size, static_cast<const double *>(m),
+                                          x_offt, y_offt, static_cast<const double *>(px),
+                                          static_cast<const double *>(py),
+                                          static_cast<const double *>(pz), st);
+    if (e != hipSuccess) return hip_fail(c, e, "move_wo_fields");
+    return LC3D_OK;
+}
+
+int lc3d_move_smart(lc3d_ctx *c, const lc3d_sort *s, const lc3d_fields6 *f, int32_t field_fp32,
+                    void *stream)
+{
+    if (!c || !sort_ok(c, s) || !f || !f->Ex || !f->Ey || !f->Ez || !f->Bx || !f->By || !f->Bz)
+        return fail(c, LC3D_EINVAL, "move_smart: bad arguments");
+    const void *fl[6] = {f->Ex, f->Ey, f->Ez, f->Bx, f->By, f->Bz};
+    return push_sort(c, s, fl, field_fp32 != 0, false, false, static_cast<hipStream_t>(stream));
+}
+
+int lc3d_deposit(lc3d_ctx *c, const lc3d_sort *s, void *ro, void *jx, void *jy, void *jz,
+                 const void *bg_ro, void *ro1, void *jx1, void *jy1, void *jz1, void *stream)
+{
+    if (!c || !sort_ok(c, s) || !ro || !jx || !jy || !jz ||
+        (bg_ro && (!ro1 || !jx1 || !jy1 || !jz1)))
+        return fail(c, LC3D_EINVAL, "deposit: bad arguments");
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    hipError_t e;
+    if (fp32(c))
+        e = launch_deposit<float, float>(push_geom(c, s->n), sort_arrays<float>(s), true, false,
+                                         bin_state(c), static_cast<float *>(ro),
+                                         static_cast<float *>(jx), static_cast<float *>(jy),
+                                         static_cast<float *>(jz), static_cast<const float *>(bg_ro),
+                                         static_cast<float *>(ro1), static_cast<float *>(jx1),
+                                         static_cast<float *>(jy1), static_cast<float *>(jz1), st);
+    else
+        e = launch_deposit<double, double>(push_geom(c, s->n), sort_arrays<double>(s), true, false,
+                                           bin_state(c), static_cast<double *>(ro),
+                                           static_cast<double *>(jx), static_cast<double *>(jy),
+                                           static_cast<double *>(jz),
+                                           static_cast<const double *>(bg_ro),
+                                           static_cast<double *>(ro1), static_cast<double *>(jx1),
+                                           static_cast<double *>(jy1), static_cast<double *>(jz1),
+                                           st);
+    if (e != hipSuccess) return hip_fail(c, e, "deposit");
+    return LC3D_OK;
+}
+
+int lc3d_compute_fields(lc3d_ctx *c, const lc3d_fields *pred, int32_t pred_fp32,
+                        const lc3d_fields *prev, const double *rbF, const double *rbP,
+                        const lc3d_currents *cp, const lc3d_currents *cf, lc3d_fields *full,
+                        lc3d_fields *half, void *stream)
+{
+    if (!c || !pred || !pred->Ex || !pred->Ey || !pred->Bx || !pred->By || !fields_ok(prev, true) ||
+        !rbF || !rbP || !currents_ok(cp) || !currents_ok(cf) || !fields_ok(full, true) ||
+        !fields_ok(half, true))
+        return fail(c, LC3D_EINVAL, "compute_fields: bad arguments");
+    return solve(c, pred, pred_fp32 != 0, prev, rbF, rbP, cp, cf, full, half, true,
+                 static_cast<hipStream_t>(stream));
+}
+
+int lc3d_plasma_step(lc3d_ctx *c, int32_t n_sorts, const lc3d_sort *sorts,
+                     const lc3d_fields *prev, const lc3d_currents *cp, const double *rbF,
+                     const double *rbP, const void *bg_ro, lc3d_fields *full, lc3d_fields *half,
+                     const lc3d_currents *cf, void *stream)
+{
+    if (!c || n_sorts < 1 || n_sorts > 2 || !sorts || !fields_ok(prev, true) ||
+        !currents_ok(cp) || !rbF || !rbP || !fields_ok(full, true) || !fields_ok(half, true) ||
+        !currents_ok(cf))
+        return fail(c, LC3D_EINVAL, "plasma_step: bad arguments");
+    for (int s = 0; s < n_sorts; ++s)
+        if (!sort_ok(c, &sorts[s])) return fail(c, LC3D_EINVAL, "plasma_step: bad sort");
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    const bool f32 = fp32(c);
+    const bool single = n_sorts == 1;
+    const void *fprev[6] = {prev->Ex, prev->Ey, prev->Ez, prev->Bx, prev->By, prev->Bz};
+    const void *fhalf[6] = {half->Ex, half->Ey, half->Ez, half->Bx, half->By, half->Bz};
+    int r;
+    // solver.py:29-42: free stream + push 1 in fields_prev, deposit 1.
+    for (int s = 0; s < n_sorts; ++s) {
+        if ((r = push_sort(c, &sorts[s], fprev, false, true, true, st))) return r;
+        if ((r = deposit_sort(c, &sorts[s], true, cf, bg_ro, single, st))) return r;
+    }
+    // solver.py:44-47: solve 1 with fields_prev as predictor; only the half fields are kept.
+    if ((r = solve(c, prev, false, prev, rbF, rbP, cp, cf, full, half, false, st))) return r;
+    // solver.py:49-57: push 2 in fields_half, deposit 2.
+    for (int s = 0; s < n_sorts; ++s) {
+        if ((r = push_sort(c, &sorts[s], fhalf, f32, false, true, st))) return r;
+        if ((r = deposit_sort(c, &sorts[s], true, cf, bg_ro, single, st))) return r;
+    }
+    // solver.py:59-62: solve 2 with fields_half as predictor.
+    if ((r = solve(c, half, f32, prev, rbF, rbP, cp, cf, full, half, true, st))) return r;
+    // solver.py:64-76: push 3 in the new fields_half, deposit 3.
+    for (int s = 0; s < n_sorts; ++s) {
+        if ((r = push_sort(c, &sorts[s], fhalf, f32, false, true, st))) return r;
+        if ((r = deposit_sort(c, &sorts[s], true, cf, bg_ro, single, st))) return r;
+    }
+    return LC3D_OK;
+}
+
+int lc3d_deposit_beam(lc3d_ctx *c, int64_t n, int64_t layer, const double *x, const double *y,
+                      const double *xi, const double *q_norm, double *ro_running,
+                      double *ro_next, void *stream)
+{
+    if (!c || n < 0 || !ro_running || !ro_next || (n > 0 && (!x || !y || !xi || !q_norm)))
+        return fail(c, LC3D_EINVAL, "deposit_beam: bad arguments");
+    const BeamGeom g{c->N, c->prm.grid_step_size, c->prm.xi_step_size, c->prm.lost_radius};
+    hipError_t e = launch_beam_deposit(g, n, layer, x, y, xi, q_norm, ro_running, ro_next, c->err,
+                                       static_cast<hipStream_t>(stream));
+    if (e != hipSuccess) return hip_fail(c, e, "deposit_beam");
+    return LC3D_OK;
+}
+
+int lc3d_beam_substep(lc3d_ctx *c, int64_t n, const double *q_m, const double *pz, double e_sub,
+                      double time_step, double *dt, int64_t *steps, void *stream)
+{
+    if (!c || n < 0 || (n > 0 && (!q_m || !pz || !dt || !steps)))
+        return fail(c, LC3D_EINVAL, "beam_substep: bad arguments");
+    hipError_t e = launch_beam_substep(n, q_m, pz, e_sub, time_step, dt, steps,
+                                       static_cast<hipStream_t>(stream));
+    if (e != hipSuccess) return hip_fail(c, e, "beam_substep");
+    return LC3D_OK;
+}
+
+int lc3d_move_beam(lc3d_ctx *c, int64_t n, int64_t beam_layer_idx, const double *q_m,
+                   const double *dt, const lc3d_fields6 *fk, const lc3d_fields6 *fk1,
+                   int64_t *steps, int64_t *id, double *x, double *y, double *xi, double *px,
+                   double *py, double *pz, uint8_t *lost, uint8_t *moved, uint8_t *fell,
+                   void *stream)
+{
+    if (!c || n < 0 || !fk || !fk1 ||
+        (n > 0 && (!q_m || !dt || !steps || !id || !x || !y || !xi || !px || !py || !pz || !lost ||
+                   !moved || !fell)))
+        return fail(c, LC3D_EINVAL, "move_beam: bad arguments");
+    const BeamGeom g{c->N, c->prm.grid_step_size, c->prm.xi_step_size, c->prm.lost_radius};
+    hipError_t e = launch_beam_move<double>(g, n, beam_layer_idx, q_m, dt, f6<double>(fk),
+                                            f6<double>(fk1), steps, id, x, y, xi, px, py, pz,
+                                            lost, moved, fell, c->err,
+                                            static_cast<hipStream_t>(stream));
+    if (e != hipSuccess) return hip_fail(c, e, "move_beam");
+    return LC3D_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,240 @@
+// Beam layer kernels (beam3d), gfx950.
+//
+// A beam layer is small (thousands of macro-particles per xi step against
+// ~10^6 plasma particles), so the beam deposit is a direct per-particle
+// scatter with float64 atomics; the push is one thread per particle running
+// the reference's CPU (explicit-midpoint) scheme with its substep loop.
+#include "common.h"
+#include "kernels.h"
+
+namespace lc3d {
+
+constexpr int MAX_BEAM_SUBSTEPS = 1 << 16;
+
+// beam3d/weights.py:40-64 with x_h and xi_loc from :152-160.  ro0 is the
+// running layer (weight xi_loc), ro1 the next one (weight 1 - xi_loc).
+__global__ __launch_bounds__(256) void k_beam_deposit(BeamGeom g, int64_t n, double xi_layer,
+                                                      const double *__restrict__ x,
+                                                      const double *__restrict__ y,
+                                                      const double *__restrict__ xi,
+                                                      const double *__restrict__ q_norm,
+                                                      double *__restrict__ ro0,
+                                                      double *__restrict__ ro1, unsigned *err)
+{
+    const int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const int N = g.N;
+    const double xh = x[k] / g.h + 0.5, yh = y[k] / g.h + 0.5;
+    if (!isfinite(xh) || !isfinite(yh)) {
+        atomicOr(err, ERR_BEAM_OFF_GRID);
+        return;
+    }
+    const double fx = floor(xh), fy = floor(yh);
+    const int ix = (int)fmin(fmax(fx + N / 2, -8.0), (double)N + 8);
+    const int iy = (int)fmin(fmax(fy + N / 2, -8.0), (double)N + 8);
+    const double xi_loc = (xi_layer - xi[k]) / g.xi_step;
+    double wx[5], wy[5];
+    weights4<double>(xh - fx - 0.5, wx);
+    weights4<double>(yh - fy - 0.5, wy);
+    const double q = q_norm[k];
+    bool off = false;
+#pragma unroll
+    for (int kx = 0; kx < 5; ++kx) {
+        const int gx = ix + kx - 2;
+#pragma unroll
+        for (int ky = 0; ky < 5; ++ky) {
+            const int gy = iy + ky - 2;
+            if (gx < 0 || gx >= N || gy < 0 || gy >= N) {
+                off = true;
+                continue;
+            }
+            const double w = wx[kx] * wy[ky];
+            atomicAdd(&ro0[gx * N + gy], q * (w * xi_loc));
+            atomicAdd(&ro1[gx * N + gy], q * (w * (1 - xi_loc)));
+        }
+    }
+    if (off) atomicOr(err, ERR_BEAM_OFF_GRID);
+}
+
+// beam3d/beam_calculator.py:18-34, 97-123: dt multiplier halved until
+// below sqrt(sqrt(1/q_m^2 + pz^2) / E_sub); dt = mult * time_step and
+// remaining_steps = int(1 / mult).
+__global__ __launch_bounds__(256) void k_beam_substep(int64_t n, const double *__restrict__ q_m,
+                                                      const double *__restrict__ pz, double e_sub,
+                                                      double time_step, double *__restrict__ dt,
+                                                      int64_t *__restrict__ steps)
+{
+    const int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const double max_dt = sqrt(sqrt(1 / (q_m[k] * q_m[k]) + pz[k] * pz[k]) / e_sub);
+    double d = 1.0;
+    // At most 2^-40: the push below also caps its substep loop.
+    for (int it = 0; it < 40 && d > max_dt; ++it) d /= 2.0;
+    dt[k] = d * time_step;
+    steps[k] = (int64_t)(1. / d);
+}
+
+template <typename TF>
+__device__ __forceinline__ void gather2(const Fields6<TF> &a, const Fields6<TF> &b, int N, int ix,
+                                        int iy, const double wx[5], const double wy[5], double w0s,
+                                        double w1s, double F[6], bool &off)
+{
+#pragma unroll
+    for (int kx = 0; kx < 5; ++kx) {
+        const int gx = ix + kx - 2;
+#pragma unroll
+        for (int ky = 0; ky < 5; ++ky) {
+            const int gy = iy + ky - 2;
+            if (gx < 0 || gx >= N || gy < 0 || gy >= N) {
+                off = true;
+                continue;
+            }
+            const int o = gx * N + gy;
+            const double w = wx[kx] * wy[ky];
+            const double w0 = w * w0s, w1 = w * w1s;
+            F[0] += (double)a.Ex[o] * w0 + (double)b.Ex[o] * w1;
+            F[1] += (double)a.Ey[o] * w0 + (double)b.Ey[o] * w1;
+            F[2] += (double)a.Ez[o] * w0 + (double)b.Ez[o] * w1;
+            F[3] += (double)a.Bx[o] * w0 + (double)b.Bx[o] * w1;
+            F[4] += (double)a.By[o] * w0 + (double)b.By[o] * w1;
+            F[5] += (double)a.Bz[o] * w0 + (double)b.Bz[o] * w1;
+        }
+    }
+}
+
+// beam3d/move.py:31-127 (the CPU scheme; its trailing "fell" test compares
+// x, not xi, as move.py:123 does).
+template <typename TF>
+__global__ __launch_bounds__(256) void k_beam_move(
+    BeamGeom g, int64_t n, double xi_k, double xi_k1, const double *__restrict__ q_m,
+    const double *__restrict__ dt, Fields6<TF> fk, Fields6<TF> fk1, int64_t *__restrict__ steps,
+    int64_t *__restrict__ id, double *__restrict__ x, double *__restrict__ y,
+    double *__restrict__ xi, double *__restrict__ px, double *__restrict__ py,
+    double *__restrict__ pz, uint8_t *__restrict__ lost, uint8_t *__restrict__ moved,
+    uint8_t *__restrict__ fell, unsigned *err)
+{
+    const int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const int N = g.N;
+    const double r2 = g.lost_radius * g.lost_radius;
+    const double qm = q_m[k], iqm = 1 / qm, d = dt[k];
+    const double sgn = qm < 0 ? -1.0 : (qm > 0 ? 1.0 : 0.0);
+    double X = x[k], Y = y[k], XI = xi[k], PX = px[k], PY = py[k], PZ = pz[k];
+    int64_t rs = steps[k];
+    bool is_lost = false, is_fell = false, off = false;
+    // Substep counts come from the caller; cap the loop so that a corrupt
+    // count cannot keep the wave running (flagged as an error).
+    int guard = 0;
+    while (rs > 0) {
+        if (++guard > MAX_BEAM_SUBSTEPS) {
+            off = true;
+            break;
+        }
+        double gm = sqrt(iqm * iqm + PX * PX + PY * PY + PZ * PZ);
+        const double xh_ = X + d / 2 * (PX / gm);
+        const double yh_ = Y + d / 2 * (PY / gm);
+        const double xih = XI + d / 2 * (PZ / gm - 1);
+        if (xi_k1 > xih) {
+            is_fell = true;
+            break;
+        }
+        if (xh_ * xh_ + yh_ * yh_ >= r2) {
+            X = xh_;
+            Y = yh_;
+            XI = xih;
+            id[k] = -id[k];
+            is_lost = true;
+            rs = 0;
+            break;
+        }
+        const double xhh = xh_ / g.h + .5, yhh = yh_ / g.h + .5;
+        const double fx = floor(xhh), fy = floor(yhh);
+        const int ix = (int)fmin(fmax(fx + N / 2, -8.0), (double)N + 8);
+        const int iy = (int)fmin(fmax(fy + N / 2, -8.0), (double)N + 8);
+        double wx[5], wy[5];
+        weights4<double>(xhh - fx - .5, wx);
+        weights4<double>(yhh - fy - .5, wy);
+        const double xi_loc = (xih - xi_k) / g.xi_step;
+        double F[6] = {0, 0, 0, 0, 0, 0};
+        gather2<TF>(fk, fk1, N, ix, iy, wx, wy, xi_loc, 1 - xi_loc, F, off);
+        const double vx = PX / gm, vy = PY / gm, vz = PZ / gm;
+        const double a = sgn * d / 2;
+        const double pxh = PX + a * (F[0] + vy * F[5] - vz * F[4]);
+        const double pyh = PY + a * (F[1] + vz * F[3] - vx * F[5]);
+        const double pzh = PZ + a * (F[2] + vx * F[4] - vy * F[3]);
+        gm = sqrt(iqm * iqm + pxh * pxh + pyh * pyh + pzh * pzh);
+        X += d * (pxh / gm);
+        Y += d * (pyh / gm);
+        XI += d * (pzh / gm - 1);
+        PX = 2 * pxh - PX;
+        PY = 2 * pyh - PY;
+        PZ = 2 * pzh - PZ;
+        if (X * X + Y * Y >= r2) {
+            id[k] = -id[k];
+            is_lost = true;
+            rs = 0;
+            break;
+        }
+        rs -= 1;
+    }
+    if (rs == 0 && xi_k1 > X) is_fell = true;
+    x[k] = X;
+    y[k] = Y;
+    xi[k] = XI;
+    px[k] = PX;
+    py[k] = PY;
+    pz[k] = PZ;
+    steps[k] = rs;
+    lost[k] = is_lost;
+    fell[k] = is_fell;
+    moved[k] = !is_fell && !is_lost;
+    if (off) atomicOr(err, ERR_BEAM_OFF_GRID);
+}
+
+hipError_t launch_beam_deposit(const BeamGeom &g, int64_t n, int64_t plasma_layer_idx,
+                               const double *x, const double *y, const double *xi,
+                               const double *q_norm, double *ro0, double *ro1, unsigned *err,
+                               hipStream_t s)
+{
+    if (n <= 0) return hipSuccess;
+    const double xi_layer = -g.xi_step * (double)plasma_layer_idx;
+    k_beam_deposit<<<(int)((n + 255) / 256), 256, 0, s>>>(g, n, xi_layer, x, y, xi, q_norm, ro0,
+                                                           ro1, err);
+    return hipGetLastError();
+}
+
+hipError_t launch_beam_substep(int64_t n, const double *q_m, const double *pz,
+                               double substepping_energy, double time_step, double *dt,
+                               int64_t *remaining_steps, hipStream_t s)
+{
+    if (n <= 0) return hipSuccess;
+    k_beam_substep<<<(int)((n + 255) / 256), 256, 0, s>>>(n, q_m, pz, substepping_energy,
+                                                           time_step, dt, remaining_steps);
+    return hipGetLastError();
+}
+
+template <typename TF>
+hipError_t launch_beam_move(const BeamGeom &g, int64_t n, int64_t beam_layer_idx,
+                            const double *q_m, const double *dt, const Fields6<TF> &fk,
+                            const Fields6<TF> &fk1, int64_t *remaining_steps, int64_t *id,
+                            double *x, double *y, double *xi, double *px, double *py, double *pz,
+                            uint8_t *lost, uint8_t *moved, uint8_t *fell, unsigned *err,
+                            hipStream_t s)
+{
+    if (n <= 0) return hipSuccess;
+    const double xi_k = (double)beam_layer_idx * -g.xi_step;
+    const double xi_k1 = (double)(beam_layer_idx + 1) * -g.xi_step;
+    k_beam_move<TF><<<(int)((n + 255) / 256), 256, 0, s>>>(g, n, xi_k, xi_k1, q_m, dt, fk, fk1,
+                                                            remaining_steps, id, x, y, xi, px, py,
+                                                            pz, lost, moved, fell, err);
+    return hipGetLastError();
+}
+
+template hipError_t launch_beam_move<double>(const BeamGeom &, int64_t, int64_t, const double *,
+                                             const double *, const Fields6<double> &,
+                                             const Fields6<double> &, int64_t *, int64_t *,
+                                             double *, double *, double *, double *, double *,
+                                             double *, uint8_t *, uint8_t *, uint8_t *,
+                                             unsigned *, hipStream_t);
+
+}  // namespace lc3d

@@ -1,0 +1,467 @@
+// Field solve of one xi step (plasma3d/fields.py:112-322), gfx950.
+//
+// The seven fields are solved together, as three batched passes over
+// N x N grids (N = 2M + 1... here M = N - 1 is the half period):
+//
+//   A  rows, forward:   right-hand side of every field computed on the fly
+//                       from the currents (fields.py:112-266) and transformed
+//                       along y (DST-I or DCT-I);
+//   B  columns:         transform along x, multiply by 1/(l_i + l_j (+c))
+//                       / (2M)^2, transform back along x -- all in LDS;
+//   C  rows, inverse:   transform back along y and form the outputs
+//                       (full = 2 half - prev for Ex Ey Bx By, half =
+//                       (full + prev)/2 for Ez Bz, fields.py:299-314);
+//   Bz finalize:        subtract the mean of Bz (fields.py:264).
+//
+// Each DST-I / DCT-I of length ~N is the real FFT of the odd / even
+// extension of period 2M, computed as one complex FFT of length M (even and
+// odd samples packed as real and imaginary parts) followed by the standard
+// split.  The complex FFT is a self-sorting Stockham FFT in LDS with radix
+// 2/4/8 stages plus generic 3/5/7/11/13 stages; one wavefront owns one
+// transform.  Twiddles exp(-i pi t / M) are computed on the host in float64.
+#include "common.h"
+#include "kernels.h"
+
+namespace lc3d {
+
+constexpr int WAVE = 64;
+constexpr int WAVES = 4;   // transforms per workgroup
+// Register arrays are sized by MAXM, the largest M a kernel instance serves
+// (256, 512 or 1024): MAXM / 64 packed samples and MAXM / 64 + 1 outputs per
+// lane.
+
+enum Field { F_EX = 0, F_EY, F_BX, F_BY, F_EZ, F_BZ, F_PHI };
+
+// Transform along y (rows) for each field: DST for fields that vanish at
+// y = edge, DCT otherwise; along x (columns) likewise.
+__device__ __forceinline__ bool dst_along_y(int f) { return f == F_EX || f == F_BY || f == F_EZ || f == F_PHI; }
+__device__ __forceinline__ bool dst_along_x(int f) { return f == F_EY || f == F_BX || f == F_EZ || f == F_PHI; }
+
+template <typename C> __device__ __forceinline__ C cmul(C a, C b) {
+    C r;
+    r.x = a.x * b.x - a.y * b.y;
+    r.y = a.x * b.y + a.y * b.x;
+    return r;
+}
+template <typename C> __device__ __forceinline__ C cadd(C a, C b) { C r; r.x = a.x + b.x; r.y = a.y + b.y; return r; }
+template <typename C> __device__ __forceinline__ C csub(C a, C b) { C r; r.x = a.x - b.x; r.y = a.y - b.y; return r; }
+template <typename C> __device__ __forceinline__ C cmul_mi(C a) { C r; r.x = a.y; r.y = -a.x; return r; }  // a * (-i)
+
+template <typename C> __device__ __forceinline__ void dft4(C &a0, C &a1, C &a2, C &a3) {
+    const C t0 = cadd(a0, a2), t1 = csub(a0, a2), t2 = cadd(a1, a3), t3 = cmul_mi(csub(a1, a3));
+    a0 = cadd(t0, t2);
+    a2 = csub(t0, t2);
+    a1 = cadd(t1, t3);
+    a3 = csub(t1, t3);
+}
+
+template <typename T, int R> struct Dft {
+    // Generic odd radix: y_q = sum_p x_p w^(pq), w = exp(-2 pi i / R).
+    using C = cplx_t<T>;
+    static __device__ __forceinline__ void run(C (&v)[R], const C *__restrict__ tw, int M) {
+        C y[R];
+        const int step = 2 * (M / R);
+#pragma unroll
+        for (int qq = 0; qq < R; ++qq) {
+            C s = v[0];
+#pragma unroll
+            for (int p = 1; p < R; ++p) s = cadd(s, cmul(v[p], tw[((p * qq) % R) * step]));
+            y[qq] = s;
+        }
+#pragma unroll
+        for (int qq = 0; qq < R; ++qq) v[qq] = y[qq];
+    }
+};
+template <typename T> struct Dft<T, 2> {
+    using C = cplx_t<T>;
+    static __device__ __forceinline__ void run(C (&v)[2], const C *, int) {
+        const C a = v[0], b = v[1];
+        v[0] = cadd(a, b);
+        v[1] = csub(a, b);
+    }
+};
+template <typename T> struct Dft<T, 4> {
+    using C = cplx_t<T>;
+    static __device__ __forceinline__ void run(C (&v)[4], const C *, int) { dft4(v[0], v[1], v[2], v[3]); }
+};
+template <typename T> struct Dft<T, 8> {
+    using C = cplx_t<T>;
+    static __device__ __forceinline__ void run(C (&v)[8], const C *, int) {
+        C e0 = v[0], e1 = v[2], e2 = v[4], e3 = v[6];
+        C o0 = v[1], o1 = v[3], o2 = v[5], o3 = v[7];
+        dft4(e0, e1, e2, e3);
+        dft4(o0, o1, o2, o3);
+        const T s = T(0.70710678118654752440);   // sqrt(1/2)
+        C w1, w3;
+        w1.x = s * (o1.x + o1.y);
+        w1.y = s * (o1.y - o1.x);                // o1 * exp(-i pi/4)
+        const C w2 = cmul_mi(o2);                // o2 * exp(-i pi/2)
+        w3.x = s * (o3.y - o3.x);
+        w3.y = -s * (o3.x + o3.y);               // o3 * exp(-3 i pi/4)
+        v[0] = cadd(e0, o0);
+        v[4] = csub(e0, o0);
+        v[1] = cadd(e1, w1);
+        v[5] = csub(e1, w1);
+        v[2] = cadd(e2, w2);
+        v[6] = csub(e2, w2);
+        v[3] = cadd(e3, w3);
+        v[7] = csub(e3, w3);
+    }
+};
+
+// One Stockham stage of radix R on an M-point sequence in LDS, one wave.
+template <typename T, int MAXM, int R>
+__device__ __forceinline__ void fft_stage(cplx_t<T> *buf, const cplx_t<T> *__restrict__ tw, int M,
+                                          int Ns, int lane)
+{
+    using C = cplx_t<T>;
+    constexpr int MAXB = (MAXM / R + WAVE - 1) / WAVE;
+    const int nb = M / R;
+    C v[MAXB][R];
+#pragma unroll
+    for (int i = 0; i < MAXB; ++i) {
+        const int b = lane + i * WAVE;
+        if (b < nb) {
+#pragma unroll
+            for (int r = 0; r < R; ++r) v[i][r] = buf[b + r * nb];
+        }
+    }
+    __syncthreads();
+    const int stride = 2 * (M / (Ns * R));
+#pragma unroll
+    for (int i = 0; i < MAXB; ++i) {
+        const int b = lane + i * WAVE;
+        if (b < nb) {
+            const int k = b % Ns;
+#pragma unroll
+            for (int r = 1; r < R; ++r) v[i][r] = cmul(v[i][r], tw[r * k * stride]);
+            Dft<T, R>::run(v[i], tw, M);
+            const int base = (b / Ns) * Ns * R + k;
+#pragma unroll
+            for (int r = 0; r < R; ++r) buf[base + r * Ns] = v[i][r];
+        }
+    }
+    __syncthreads();
+}
+
+template <typename T, int MAXM>
+__device__ void fft(cplx_t<T> *buf, const cplx_t<T> *__restrict__ tw, const FftPlan &pl, int lane)
+{
+    int Ns = 1;
+    for (int s = 0; s < pl.ns; ++s) {
+        const int R = pl.radix[s];
+        switch (R) {
+        case 2: fft_stage<T, MAXM, 2>(buf, tw, pl.M, Ns, lane); break;
+        case 3: fft_stage<T, MAXM, 3>(buf, tw, pl.M, Ns, lane); break;
+        case 4: fft_stage<T, MAXM, 4>(buf, tw, pl.M, Ns, lane); break;
+        case 5: fft_stage<T, MAXM, 5>(buf, tw, pl.M, Ns, lane); break;
+        case 7: fft_stage<T, MAXM, 7>(buf, tw, pl.M, Ns, lane); break;
+        case 8: fft_stage<T, MAXM, 8>(buf, tw, pl.M, Ns, lane); break;
+        case 11: fft_stage<T, MAXM, 11>(buf, tw, pl.M, Ns, lane); break;
+        case 13: fft_stage<T, MAXM, 13>(buf, tw, pl.M, Ns, lane); break;
+        default: break;
+        }
+        Ns *= R;
+    }
+}
+
+// Unnormalised DST-I (x(1..M-1) -> Y(1..M-1)) or DCT-I (x(0..M) -> Y(0..M))
+// of one sequence, computed by one wave.  load(t) gives x(t); store(k, Y)
+// receives the result.  `active` false runs the barriers only.
+template <typename T, int MAXM, typename Load, typename Store>
+__device__ void r2r(bool dst, cplx_t<T> *buf, const cplx_t<T> *__restrict__ tw, const FftPlan &pl,
+                    int lane, bool active, Load load, Store store)
+{
+    using C = cplx_t<T>;
+    constexpr int MAXN = MAXM / WAVE;
+    constexpr int MAXK = MAXM / WAVE + 1;
+    const int M = pl.M;
+    // Pack the period-2M extension: z[n] = e[2n] + i e[2n+1].
+    C z[MAXN];
+#pragma unroll
+    for (int i = 0; i < MAXN; ++i) {
+        const int n = lane + i * WAVE;
+        if (active && n < M) {
+            T e[2];
+#pragma unroll
+            for (int u = 0; u < 2; ++u) {
+                const int tt = 2 * n + u;
+                if (dst) {
+                    e[u] = (tt == 0 || tt == M) ? T(0) : (tt < M ? load(tt) : -load(2 * M - tt));
+                } else {
+                    e[u] = tt <= M ? load(tt) : load(2 * M - tt);
+                }
+            }
+            z[i].x = e[0];
+            z[i].y = e[1];
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < MAXN; ++i) {
+        const int n = lane + i * WAVE;
+        if (active && n < M) buf[n] = z[i];
+    }
+    __syncthreads();
+    fft<T, MAXM>(buf, tw, pl, lane);
+    // Split: E[k] = (Z[k] + Z*[M-k])/2 + W^k (Z[k] - Z*[M-k])/(2i), W = exp(-i pi/M).
+    T y[MAXK];
+#pragma unroll
+    for (int i = 0; i < MAXK; ++i) {
+        const int k = lane + i * WAVE;
+        y[i] = 0;
+        if (active && k <= M) {
+            const C zk = buf[k == M ? 0 : k];
+            const C zm = buf[k == 0 ? 0 : M - k];
+            const T ar = T(0.5) * (zk.x + zm.x), ai = T(0.5) * (zk.y - zm.y);
+            const T br = T(0.5) * (zk.y + zm.y), bi = T(-0.5) * (zk.x - zm.x);
+            const C w = tw[k];
+            const T er = ar + (w.x * br - w.y * bi);
+            const T ei = ai + (w.x * bi + w.y * br);
+            y[i] = dst ? -ei : er;
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < MAXK; ++i) {
+        const int k = lane + i * WAVE;
+        if (active && k <= M && (!dst || (k > 0 && k < M))) store(k, y[i]);
+    }
+    __syncthreads();
+}
+
+// ---- pass A: right-hand sides (fields.py:112-266) + forward transform along y ----
+template <typename TG, typename TPRED> struct RhsEval {
+    const SolveIn<TG, TPRED> &in;
+    int N;
+    TG inv2h, invdxi, c;
+    __device__ TG roF(int o) const { return in.roF[0][o] + in.roF[1][o]; }
+    __device__ TG jxF(int o) const { return in.jxF[0][o] + in.jxF[1][o]; }
+    __device__ TG jyF(int o) const { return in.jyF[0][o] + in.jyF[1][o]; }
+    __device__ TG jzF(int o) const { return in.jzF[0][o] + in.jzF[1][o]; }
+    __device__ TG roP(int o) const { return in.roP[0][o] + in.roP[1][o]; }
+    __device__ TG jxP(int o) const { return in.jxP[0][o] + in.jxP[1][o]; }
+    __device__ TG jyP(int o) const { return in.jyP[0][o] + in.jyP[1][o]; }
+    __device__ TG jzP(int o) const { return in.jzP[0][o] + in.jzP[1][o]; }
+    __device__ TG ro_half(int o) const {
+        return (roF(o) + (TG)in.rbF[o] + roP(o) + (TG)in.rbP[o]) * TG(0.5);
+    }
+    __device__ TG jz_half(int o) const {
+        return (jzF(o) + (TG)in.rbF[o] + jzP(o) + (TG)in.rbP[o]) * TG(0.5);
+    }
+
+    __device__ TG operator()(int f, int i, int j) const {
+        const int o = i * N + j;
+        const bool inner = i > 0 && i < N - 1 && j > 0 && j < N - 1;
+        switch (f) {
+        case F_EX: {
+            const TG d = inner ? (ro_half(o + N) - ro_half(o - N)) * inv2h : TG(0);
+            return -((d - (jxP(o) - jxF(o)) * invdxi) - c * (TG)in.ExP[o]);
+        }
+        case F_EY: {
+            const TG d = inner ? (ro_half(o + 1) - ro_half(o - 1)) * inv2h : TG(0);
+            return -((d - (jyP(o) - jyF(o)) * invdxi) - c * (TG)in.EyP[o]);
+        }
+        case F_BX: {
+            const TG d = inner ? (jz_half(o + 1) - jz_half(o - 1)) * inv2h : TG(0);
+            return (d - (jyP(o) - jyF(o)) * invdxi) + c * (TG)in.BxP[o];
+        }
+        case F_BY: {
+            const TG d = inner ? (jz_half(o + N) - jz_half(o - N)) * inv2h : TG(0);
+            return -((d - (jxP(o) - jxF(o)) * invdxi) - c * (TG)in.ByP[o]);
+        }
+        case F_EZ:
+            return -((jxF(o + N) - jxF(o - N)) + (jyF(o + 1) - jyF(o - 1))) * inv2h;
+        case F_BZ: {
+            const TG a = inner ? jxF(o + 1) - jxF(o - 1) : TG(0);
+            const TG b = inner ? jyF(o + N) - jyF(o - N) : TG(0);
+            return -(a - b) * inv2h;
+        }
+        default:   // F_PHI
+            return roF(o) - jzF(o);
+        }
+    }
+};
+
+__device__ __forceinline__ bool has_spectral_row(int f, int i, int N) {
+    // Rows that carry data after the transform along y: all rows for the
+    // fields whose x-transform is a DCT, the interior rows otherwise.
+    return dst_along_x(f) ? (i > 0 && i < N - 1) : (i >= 0 && i < N);
+}
+
+template <typename TG, typename TPRED, int MAXM>
+__global__ __launch_bounds__(256) void k_rows_fwd(SolveGeom g, SolveIn<TG, TPRED> in, int nrowblk)
+{
+    using C = cplx_t<TG>;
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    const int wave = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
+    C *buf = reinterpret_cast<C *>(lds_raw) + wave * g.plan.M;
+    const int f = blockIdx.y;
+    const int i = blockIdx.x * WAVES + wave;
+    const int N = g.N;
+    const bool active = i < N && has_spectral_row(f, i, N);
+    RhsEval<TG, TPRED> rhs{in, N, TG(1.0 / (2.0 * g.h)), TG(1.0 / g.xi_step), TG(g.c)};
+    TG *out = reinterpret_cast<TG *>(g.work) + (size_t)f * N * N + (size_t)i * N;
+    (void)nrowblk;
+    r2r<TG, MAXM>(dst_along_y(f), buf, reinterpret_cast<const C *>(g.tw), g.plan, lane, active,
+            [&](int t) { return rhs(f, i, t); }, [&](int k, TG y) { out[k] = y; });
+}
+
+// ---- pass B: columns, forward x-transform, multiply, inverse x-transform ------
+template <typename TG, int MAXM>
+__global__ __launch_bounds__(256) void k_cols(SolveGeom g)
+{
+    using C = cplx_t<TG>;
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    const int wave = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
+    C *buf = reinterpret_cast<C *>(lds_raw) + wave * g.plan.M;
+    const int f = blockIdx.y;
+    const int N = g.N;
+    const int j = xcd_swizzle(blockIdx.x, gridDim.x) * WAVES + wave;   // spectral column (ky)
+    const bool ydst = dst_along_y(f), xdst = dst_along_x(f);
+    const bool active = ydst ? (j > 0 && j < N - 1) : (j < N);
+    TG *col = reinterpret_cast<TG *>(g.work) + (size_t)f * N * N + j;
+    const double lj = active ? g.lam[j] : 0.0;
+    const double cc = (f == F_EZ || f == F_BZ || f == F_PHI) ? 0.0 : g.c;
+    const double d2 = 4.0 * (double)(N - 1) * (double)(N - 1);
+    TG *bre = reinterpret_cast<TG *>(buf);
+    // Forward along x, multiplied by the solver matrix (initialization.py:15-67)
+    // on the way out; the spectrum is parked in the real view of the buffer.
+    r2r<TG, MAXM>(xdst, buf, reinterpret_cast<const C *>(g.tw), g.plan, lane, active,
+            [&](int t) { return col[(size_t)t * N]; },
+            [&](int k, TG y) {
+                const double den = g.lam[k] + lj + cc;
+                const double mul = (f == F_BZ && k == 0 && j == 0) ? 0.0 : (1.0 / den) / d2;
+                bre[k] = (TG)((double)y * mul);
+            });
+    r2r<TG, MAXM>(xdst, buf, reinterpret_cast<const C *>(g.tw), g.plan, lane, active,
+            [&](int t) { return bre[t]; }, [&](int k, TG y) { col[(size_t)k * N] = y; });
+}
+
+// ---- pass C: inverse along y and output assembly --------------------------------
+template <typename TG, typename TFULL, typename THALF, int MAXM>
+__global__ __launch_bounds__(256) void k_rows_inv(SolveGeom g, SolveOut<TFULL, THALF> out,
+                                                  bool write_full)
+{
+    using C = cplx_t<TG>;
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    const int wave = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
+    C *buf = reinterpret_cast<C *>(lds_raw) + wave * g.plan.M;
+    const int f = blockIdx.y;
+    const int i = blockIdx.x * WAVES + wave;
+    const int N = g.N;
+    const bool row_ok = i < N;
+    const bool spec = row_ok && has_spectral_row(f, i, N);
+    const TG *src = reinterpret_cast<const TG *>(g.work) + (size_t)f * N * N + (size_t)i * N;
+    const size_t ro = (size_t)i * N;
+    const TFULL *prev = out.prev[f];
+    TFULL *full = out.full[f];
+    THALF *half = out.half[f];
+
+    auto emit = [&](int jj, TG r) {
+        const size_t o = ro + jj;
+        switch (f) {
+        case F_EX: case F_EY: case F_BX: case F_BY: {
+            half[o] = (THALF)r;
+            if (write_full) full[o] = (TFULL)(2 * (TFULL)r - prev[o]);
+            break;
+        }
+        case F_EZ: {
+            const TFULL v = (TFULL)r;
+            if (write_full) full[o] = v;
+            half[o] = (THALF)((v + prev[o]) / 2);
+            break;
+        }
+        case F_PHI:
+            full[o] = (TFULL)r;   // fields_half.Phi is fields_full.Phi
+            half[o] = (THALF)r;
+            break;
+        default:                  // F_BZ: raw values, mean removed by k_bz_finalize
+            full[o] = (TFULL)r;
+            break;
+        }
+    };
+
+    double s = 0;
+    const bool ydst = dst_along_y(f);
+    r2r<TG, MAXM>(ydst, buf, reinterpret_cast<const C *>(g.tw), g.plan, lane, spec,
+            [&](int t) { return src[t]; },
+            [&](int k, TG y) {
+                emit(k, y);
+                s += (double)y;
+            });
+    if (row_ok && !spec) {
+        for (int jj = lane; jj < N; jj += WAVE) emit(jj, TG(0));   // zero rows of the solve
+    } else if (spec && ydst && lane == 0) {
+        emit(0, TG(0));
+        emit(N - 1, TG(0));
+    }
+    if (f == F_BZ && row_ok) {
+#pragma unroll
+        for (int off = WAVE / 2; off > 0; off >>= 1) s += __shfl_xor(s, off);
+        if (lane == 0) g.rowsum[i] = s;
+    }
+}
+
+template <typename TFULL, typename THALF>
+__global__ __launch_bounds__(256) void k_bz_finalize(int N, const double *__restrict__ rowsum,
+                                                     const TFULL *__restrict__ prev,
+                                                     TFULL *__restrict__ full,
+                                                     THALF *__restrict__ half)
+{
+    __shared__ double red[256];
+    double s = 0;
+    for (int i = threadIdx.x; i < N; i += 256) s += rowsum[i];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int off = 128; off > 0; off >>= 1) {
+        if ((int)threadIdx.x < off) red[threadIdx.x] += red[threadIdx.x + off];
+        __syncthreads();
+    }
+    const double mean = red[0] / ((double)N * N);
+    const int64_t n = (int64_t)N * N;
+    for (int64_t o = blockIdx.x * (int64_t)256 + threadIdx.x; o < n; o += (int64_t)gridDim.x * 256) {
+        const TFULL v = (TFULL)((double)full[o] - mean);
+        full[o] = v;
+        half[o] = (THALF)((v + prev[o]) / 2);
+    }
+}
+
+template <typename TG, typename TPRED, typename TFULL, typename THALF>
+hipError_t launch_compute_fields(const SolveGeom &g, const SolveIn<TG, TPRED> &in,
+                                 const SolveOut<TFULL, THALF> &out, bool write_full, hipStream_t s)
+{
+    const int N = g.N;
+    const size_t lds = (size_t)WAVES * g.plan.M * sizeof(cplx_t<TG>);
+    const dim3 rows((N + WAVES - 1) / WAVES, 7);
+    const int M = g.plan.M;
+    if (M <= 256) {
+        k_rows_fwd<TG, TPRED, 256><<<rows, 256, lds, s>>>(g, in, rows.x);
+        k_cols<TG, 256><<<rows, 256, lds, s>>>(g);
+        k_rows_inv<TG, TFULL, THALF, 256><<<rows, 256, lds, s>>>(g, out, write_full);
+    } else if (M <= 512) {
+        k_rows_fwd<TG, TPRED, 512><<<rows, 256, lds, s>>>(g, in, rows.x);
+        k_cols<TG, 512><<<rows, 256, lds, s>>>(g);
+        k_rows_inv<TG, TFULL, THALF, 512><<<rows, 256, lds, s>>>(g, out, write_full);
+    } else {
+        k_rows_fwd<TG, TPRED, 1024><<<rows, 256, lds, s>>>(g, in, rows.x);
+        k_cols<TG, 1024><<<rows, 256, lds, s>>>(g);
+        k_rows_inv<TG, TFULL, THALF, 1024><<<rows, 256, lds, s>>>(g, out, write_full);
+    }
+    const int fb = (int)std::min<int64_t>(1024, ((int64_t)N * N + 255) / 256);
+    k_bz_finalize<TFULL, THALF><<<fb, 256, 0, s>>>(N, g.rowsum, out.prev[F_BZ], out.full[F_BZ],
+                                                   out.half[F_BZ]);
+    return hipGetLastError();
+}
+
+template hipError_t launch_compute_fields<double, double, double, double>(
+    const SolveGeom &, const SolveIn<double, double> &, const SolveOut<double, double> &, bool,
+    hipStream_t);
+template hipError_t launch_compute_fields<float, double, double, float>(
+    const SolveGeom &, const SolveIn<float, double> &, const SolveOut<double, float> &, bool,
+    hipStream_t);
+template hipError_t launch_compute_fields<float, float, double, float>(
+    const SolveGeom &, const SolveIn<float, float> &, const SolveOut<double, float> &, bool,
+    hipStream_t);
+
+}  // namespace lc3d

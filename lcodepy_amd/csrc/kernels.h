@@ -1,0 +1,124 @@
+// Host-side launch interfaces of the lcodepy_amd kernels (gfx950).
+//
+// TP is the bulk particle type (px, py, pz, q, m): double in the FP64 form,
+// float in the FP32-bulk form.  Positions (x_init, y_init, x_offt, y_offt)
+// are always double.  TF is the type of the field arrays a push reads, TG the
+// type of the current / spectral grids.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+namespace lc3d {
+
+template <typename T> struct Fields6 {
+    const T *Ex, *Ey, *Ez, *Bx, *By, *Bz;
+};
+
+struct PushGeom {
+    int64_t n;        // particles in this sort
+    int N;            // grid steps
+    double h;         // transverse step
+    double xi_step;
+    double boundary;  // plasma reflection boundary
+};
+
+// Per-sort particle arrays of the previous (k) and next (k+1) layer.
+template <typename TP> struct SortArrays {
+    const double *x_init, *y_init;
+    const TP *m;
+    const double *xo_prev, *yo_prev;
+    const TP *px_prev, *py_prev, *pz_prev, *q_prev;
+    double *xo, *yo;
+    TP *px, *py, *pz, *q;
+};
+
+// Binning state shared by push and deposit (owned by the context).
+struct BinState {
+    int *count;       // N*N, zero between deposits
+    int *start;       // N*N + 1
+    int *blocks;      // scan block sums
+    int *key;         // per particle: cell index or -1
+    int *idx;         // per particle: particle ids sorted by cell
+    unsigned *err;    // sticky error bits
+};
+
+// plasma3d/move.py:151-165 (free-streaming estimate).
+template <typename TP>
+hipError_t launch_move_wo_fields(int64_t n, double xi_step, const TP *m, double *xo, double *yo,
+                                 const TP *px, const TP *py, const TP *pz, hipStream_t s);
+
+// plasma3d/move.py:52-148.  first=true folds in the free-streaming estimate
+// from the previous layer (move.py:331-349); bin=true also records each
+// particle's deposit cell for the deposit that follows.
+template <typename TP, typename TF>
+hipError_t launch_push(const PushGeom &g, const Fields6<TF> &f, const SortArrays<TP> &a,
+                       bool first, bool bin, const BinState &bs, hipStream_t s);
+
+// plasma3d/weights.py:35-67, 249-285 for one sort.  When `binned` is false
+// the cells are computed here; otherwise the preceding push recorded them.
+// bg_ro != nullptr also writes the background-ion column (rhoj.py:17-18)
+// into ro1/jx1/jy1/jz1.
+template <typename TP, typename TG>
+hipError_t launch_deposit(const PushGeom &g, const SortArrays<TP> &a, bool use_full, bool binned,
+                          const BinState &bs, TG *ro, TG *jx, TG *jy, TG *jz, const TG *bg_ro,
+                          TG *ro1, TG *jx1, TG *jy1, TG *jz1, hipStream_t s);
+
+// ---- field solve (plasma3d/fields.py:112-322) -------------------------------
+struct FftPlan {
+    int M;            // complex length N - 1
+    int ns;           // stages
+    int radix[12];
+};
+
+template <typename TG, typename TPRED> struct SolveIn {
+    const TG *roF[2], *jxF[2], *jyF[2], *jzF[2];   // currents of layer k+1, per column
+    const TG *roP[2], *jxP[2], *jyP[2], *jzP[2];   // currents of layer k
+    const double *rbF, *rbP;                       // beam charge density k+1, k
+    const TPRED *ExP, *EyP, *BxP, *ByP;            // Helmholtz predictor
+};
+
+template <typename TFULL, typename THALF> struct SolveOut {
+    const TFULL *prev[7];   // Ex Ey Bx By Ez Bz Phi of layer k
+    TFULL *full[7];
+    THALF *half[7];
+};
+
+struct SolveGeom {
+    int N;
+    double h, xi_step, c;   // c: subtraction coefficient
+    FftPlan plan;
+    const double *lam;      // N eigenvalues 4/h^2 sin^2(k pi / 2(N-1))
+    const void *tw;         // 2(N-1) roots exp(-i pi t / (N-1)), complex TG
+    void *work;             // 7 N*N TG spectral scratch
+    double *rowsum;         // N
+};
+
+template <typename TG, typename TPRED, typename TFULL, typename THALF>
+hipError_t launch_compute_fields(const SolveGeom &g, const SolveIn<TG, TPRED> &in,
+                                 const SolveOut<TFULL, THALF> &out, bool write_full,
+                                 hipStream_t s);
+
+// ---- beam (beam3d) -----------------------------------------------------------
+struct BeamGeom {
+    int N;
+    double h, xi_step, lost_radius;
+};
+
+hipError_t launch_beam_deposit(const BeamGeom &g, int64_t n, int64_t plasma_layer_idx,
+                               const double *x, const double *y, const double *xi,
+                               const double *q_norm, double *ro0, double *ro1, unsigned *err,
+                               hipStream_t s);
+
+hipError_t launch_beam_substep(int64_t n, const double *q_m, const double *pz,
+                               double substepping_energy, double time_step, double *dt,
+                               int64_t *remaining_steps, hipStream_t s);
+
+template <typename TF>
+hipError_t launch_beam_move(const BeamGeom &g, int64_t n, int64_t beam_layer_idx,
+                            const double *q_m, const double *dt, const Fields6<TF> &fk,
+                            const Fields6<TF> &fk1, int64_t *remaining_steps, int64_t *id,
+                            double *x, double *y, double *xi, double *px, double *py, double *pz,
+                            uint8_t *lost, uint8_t *moved, uint8_t *fell, unsigned *err,
+                            hipStream_t s);
+
+}  // namespace lc3d

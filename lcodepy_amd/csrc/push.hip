@@ -1,0 +1,245 @@
+// Plasma particle push for one xi step (gfx950).
+//
+// One thread per macro-particle over structure-of-arrays buffers, so every
+// load and store of the particle state is a fully coalesced 64-lane access.
+// The push follows plasma3d/move.py:52-148: midpoint position (reflected at
+// the wall), 5x5 quartic-spline gather of six fields, two fixed-point passes
+// of the relativistic momentum update, position update from the half-step
+// momentum, wall reflection and absorption.  In the FP32-bulk form the
+// momentum arithmetic runs in float while gamma, the 1/(1 - v_z) factor and
+// the positions stay double.
+#include "common.h"
+#include "kernels.h"
+
+namespace lc3d {
+
+template <typename TP>
+__global__ __launch_bounds__(256) void k_move_wo_fields(int64_t n, double xi_step,
+                                                        const TP *__restrict__ m,
+                                                        double *__restrict__ xo,
+                                                        double *__restrict__ yo,
+                                                        const TP *__restrict__ px,
+                                                        const TP *__restrict__ py,
+                                                        const TP *__restrict__ pz)
+{
+    const int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const double mk = m[k];
+    if (mk == 0) return;
+    const double a = px[k], b = py[k], c = pz[k];
+    const double g = sqrt(mk * mk + c * c + a * a + b * b);
+    const double d = g - c;
+    xo[k] += a / d * xi_step;
+    yo[k] += b / d * xi_step;
+}
+
+// Reflection of a position at +-b (plasma3d/move.py:28-49).  Returns true
+// when any reflection happened (the caller then zeroes the momenta).
+__device__ __forceinline__ bool reflect(double x_init, double y_init, double b, double &x,
+                                        double &y, double &xo, double &yo)
+{
+    bool hit = false;
+    if (x > b) { x = 2 * b - x; xo = x - x_init; hit = true; }
+    if (x < -b) { x = -2 * b - x; xo = x - x_init; hit = true; }
+    if (y > b) { y = 2 * b - y; yo = y - y_init; hit = true; }
+    if (y < -b) { y = -2 * b - y; yo = y - y_init; hit = true; }
+    return hit;
+}
+
+// Cell of the deposit stencil centre for a position (weights.py:53-56); -1
+// (with an error bit) when the 5x5 stencil would leave the grid or the
+// position is not finite.
+__device__ __forceinline__ int deposit_cell(double x, double y, double h, int N, unsigned *err)
+{
+    int ix, iy;
+    double xl, yl;
+    bool finite;
+    if (!stencil_cell(x, y, h, N, ix, iy, xl, yl, finite)) {
+        atomicOr(err, finite ? ERR_PARTICLE_OFF_GRID : ERR_NONFINITE);
+        return -1;
+    }
+    return ix * N + iy;
+}
+
+template <typename TP, typename TF, bool FIRST, bool BIN>
+__global__ __launch_bounds__(256) void k_push(PushGeom g, Fields6<TF> f, SortArrays<TP> a,
+                                              BinState bs)
+{
+    const int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (k >= g.n) return;
+    const int N = g.N;
+    const double b = g.boundary, dxi = g.xi_step;
+
+    const double xi0 = a.x_init[k], yi0 = a.y_init[k];
+    const double xop = a.xo_prev[k], yop = a.yo_prev[k];
+    const TP opx = a.px_prev[k], opy = a.py_prev[k], opz = a.pz_prev[k];
+    const double m = (double)a.m[k];
+
+    TP q;
+    double xof, yof;
+    if (FIRST) {
+        // particles_full = copy(particles_prev) + free-streaming estimate
+        q = a.q_prev[k];
+        xof = xop;
+        yof = yop;
+        if (m != 0) {
+            const double A = opx, B = opy, C = opz;
+            const double gm = sqrt(m * m + C * C + A * A + B * B);
+            xof = xop + A / (gm - C) * dxi;
+            yof = yop + B / (gm - C) * dxi;
+        }
+    } else {
+        q = a.q[k];
+        xof = a.xo[k];
+        yof = a.yo[k];
+    }
+
+    if (q == TP(0)) {
+        if (FIRST) {
+            a.xo[k] = xof;
+            a.yo[k] = yof;
+            a.px[k] = opx;
+            a.py[k] = opy;
+            a.pz[k] = opz;
+            a.q[k] = q;
+        }
+        if (BIN) bs.key[k] = -1;
+        return;
+    }
+
+    // Midpoint between the previous position and the current estimate.
+    double x = xi0 + (xop + xof) * 0.5, y = yi0 + (yop + yof) * 0.5;
+    double xo_tmp = 0, yo_tmp = 0;
+    TP px = opx, py = opy, pz = opz;
+    if (reflect(xi0, yi0, b, x, y, xo_tmp, yo_tmp)) px = py = pz = TP(0);
+
+    // 5x5 gather of the six fields at the midpoint (move.py:90-107).
+    TP Ex = 0, Ey = 0, Ez = 0, Bx = 0, By = 0, Bz = 0;
+    {
+        int ix, iy;
+        double xl, yl;
+        bool finite;
+        if (stencil_cell(x, y, g.h, N, ix, iy, xl, yl, finite)) {
+            TP wx[5], wy[5];
+            weights4<TP>((TP)xl, wx);
+            weights4<TP>((TP)yl, wy);
+#pragma unroll
+            for (int kx = 0; kx < 5; ++kx) {
+                const int row = (ix + kx - 2) * N + iy - 2;
+#pragma unroll
+                for (int ky = 0; ky < 5; ++ky) {
+                    const TP w = wx[kx] * wy[ky];
+                    const int id = row + ky;
+                    Ex += (TP)f.Ex[id] * w;
+                    Bx += (TP)f.Bx[id] * w;
+                    Ey += (TP)f.Ey[id] * w;
+                    By += (TP)f.By[id] * w;
+                    Ez += (TP)f.Ez[id] * w;
+                    Bz += (TP)f.Bz[id] * w;
+                }
+            }
+        } else {
+            // Outside the grid (or not finite): no field is gathered.
+            atomicOr(bs.err, finite ? ERR_PARTICLE_OFF_GRID : ERR_NONFINITE);
+        }
+    }
+
+    // Two fixed-point passes of the momentum update (move.py:110-125).
+    const double qd = (double)q * dxi;
+    TP dpx, dpy, dpz;
+#pragma unroll
+    for (int pass = 0; pass < 2; ++pass) {
+        const double P = px, Q = py, R = pz;
+        const double gm = sqrt(m * m + R * R + P * P + Q * Q);
+        const TP vx = (TP)(P / gm), vy = (TP)(Q / gm), vz = (TP)(R / gm);
+        const TP factor = (TP)(qd / (1 - R / gm));
+        dpx = factor * (Ex + vy * Bz - vz * By);
+        dpy = factor * (Ey - vx * Bz + vz * Bx);
+        dpz = factor * (Ez + vx * By - vy * Bx);
+        px = opx + dpx * TP(0.5);
+        py = opy + dpy * TP(0.5);
+        pz = opz + dpz * TP(0.5);
+    }
+
+    // Position from the half-step momentum, then the full-step momentum.
+    double xo, yo;
+    {
+        const double P = px, Q = py, R = pz;
+        const double gm = sqrt(m * m + R * R + P * P + Q * Q);
+        const double d = gm - R;
+        xo = xop + P / d * dxi;
+        yo = yop + Q / d * dxi;
+    }
+    px = opx + dpx;
+    py = opy + dpy;
+    pz = opz + dpz;
+
+    x = xi0 + xo;
+    y = yi0 + yo;
+    if (reflect(xi0, yi0, b, x, y, xo, yo)) px = py = pz = TP(0);
+    x = xi0 + xo;
+    y = yi0 + yo;
+    bool absorbed = false;
+    if (x >= b || x <= -b || y >= b || y <= -b) {
+        xo = yo = 0;
+        px = py = pz = TP(0);
+        absorbed = true;
+    }
+
+    a.xo[k] = xo;
+    a.yo[k] = yo;
+    a.px[k] = px;
+    a.py[k] = py;
+    a.pz[k] = pz;
+    if (FIRST || absorbed) a.q[k] = absorbed ? TP(0) : q;
+
+    if (BIN) {
+        int key = -1;
+        if (!absorbed) {
+            key = deposit_cell(xi0 + xo, yi0 + yo, g.h, N, bs.err);
+            if (key >= 0) atomicAdd(&bs.count[key], 1);
+        }
+        bs.key[k] = key;
+    }
+}
+
+template <typename TP>
+hipError_t launch_move_wo_fields(int64_t n, double xi_step, const TP *m, double *xo, double *yo,
+                                 const TP *px, const TP *py, const TP *pz, hipStream_t s)
+{
+    if (n <= 0) return hipSuccess;
+    const int blocks = (int)((n + 255) / 256);
+    k_move_wo_fields<TP><<<blocks, 256, 0, s>>>(n, xi_step, m, xo, yo, px, py, pz);
+    return hipGetLastError();
+}
+
+template <typename TP, typename TF>
+hipError_t launch_push(const PushGeom &g, const Fields6<TF> &f, const SortArrays<TP> &a,
+                       bool first, bool bin, const BinState &bs, hipStream_t s)
+{
+    if (g.n <= 0) return hipSuccess;
+    const int blocks = (int)((g.n + 255) / 256);
+    if (first && bin) k_push<TP, TF, true, true><<<blocks, 256, 0, s>>>(g, f, a, bs);
+    else if (first) k_push<TP, TF, true, false><<<blocks, 256, 0, s>>>(g, f, a, bs);
+    else if (bin) k_push<TP, TF, false, true><<<blocks, 256, 0, s>>>(g, f, a, bs);
+    else k_push<TP, TF, false, false><<<blocks, 256, 0, s>>>(g, f, a, bs);
+    return hipGetLastError();
+}
+
+template hipError_t launch_move_wo_fields<double>(int64_t, double, const double *, double *,
+                                                  double *, const double *, const double *,
+                                                  const double *, hipStream_t);
+template hipError_t launch_move_wo_fields<float>(int64_t, double, const float *, double *,
+                                                 double *, const float *, const float *,
+                                                 const float *, hipStream_t);
+template hipError_t launch_push<double, double>(const PushGeom &, const Fields6<double> &,
+                                                const SortArrays<double> &, bool, bool,
+                                                const BinState &, hipStream_t);
+template hipError_t launch_push<float, double>(const PushGeom &, const Fields6<double> &,
+                                               const SortArrays<float> &, bool, bool,
+                                               const BinState &, hipStream_t);
+template hipError_t launch_push<float, float>(const PushGeom &, const Fields6<float> &,
+                                              const SortArrays<float> &, bool, bool,
+                                              const BinState &, hipStream_t);
+
+}  // namespace lc3d
