@@ -1,0 +1,273 @@
+#!/usr/bin/env python
+"""xi-steps/s of lcodePy's quasistatic 3D xi step on one MI355X.
+
+Workload (the 513 x 513, 4 ppc configuration): transverse window 25.6
+(step 0.05, N = 513), 4 plasma macro-particles per cell (964^2 = 929,296)
+with seeded thermal momenta ~ N(0, 1e-3), background ions, xi_step 0.02;
+a 1e6-particle seeded electron driver, n_b/n0 = 1, sigma_r = 0.5,
+sigma_xi = 1 (truncated at +-3 sigma), gamma ~ N(1000, 10).  One "step" is
+one xi step of the whole path: beam layer deposit, the plasma step (free
+stream, 3 pushes, 3 deposits, 2 field solves) and the beam push, with all
+inputs resident in HBM.  The FP32-bulk form is the headline value; the
+float64 form runs on the same inputs for the speed ratio and the on-axis Ez
+error.  Synthetic data (seeded), no files read.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+METRIC = ('xi-steps/s at 513² 4ppc, FP32-bulk vs FP64 @Ez err≤1e-4; HBM TB/s')
+HBM_PEAK = 8.0e12
+
+
+def workload_config(precision: str) -> dict:
+    return {'geometry': '3d', 'processing-unit-type': 'mi355x', 'precision': precision,
+            'window-width': 12.8, 'transverse-step': 0.05,
+            'window-length': 20.0, 'xi-step': 0.02, 'time-limit': 1.0, 'time-step': 1.0,
+            'ion-model': 'background', 'plasma-particles-per-cell': 4}
+
+
+def driver_beam(n_particles: int, xi_step: float, seed: int = 1) -> np.ndarray:
+    """Seeded Gaussian driver, head at xi = 0 (particle_dtype3d records)."""
+    from lcodepy_amd.beam_generator import particle_dtype3d
+    rng = np.random.RandomState(seed)
+    sig_r, sig_xi, n_b, xi0 = 0.5, 1.0, 1.0, -3.0
+    xi = np.empty(0)
+    while xi.size < n_particles:
+        d = rng.normal(0.0, sig_xi, 2 * n_particles)
+        xi = np.concatenate([xi, xi0 + d[np.abs(d) <= 3 * sig_xi]])
+    xi = xi[:n_particles]
+    b = np.zeros(n_particles, dtype=particle_dtype3d)
+    b['xi'] = xi
+    b['x'] = rng.normal(0.0, sig_r, n_particles)
+    b['y'] = rng.normal(0.0, sig_r, n_particles)
+    b['px'] = rng.normal(0.0, 1e-2, n_particles)
+    b['py'] = rng.normal(0.0, 1e-2, n_particles)
+    b['pz'] = rng.normal(1000.0, 10.0, n_particles)
+    b['q_m'] = -1.0
+    b['q_norm'] = -n_b * 2 * math.pi * sig_r ** 2 * math.sqrt(2 * math.pi) * sig_xi / (
+        n_particles * xi_step)
+    b['id'] = np.arange(1, n_particles + 1)
+    return b[np.argsort(-b['xi'])]
+
+
+def thermalize(particles: dict, seed: int = 2, sigma: float = 1e-3) -> None:
+    import torch
+    rng = np.random.default_rng(seed)
+    for arrs in particles.values():
+        for k in ('px', 'py', 'pz'):
+            t = getattr(arrs, k)
+            noise = rng.normal(0.0, sigma, tuple(t.shape))
+            t.copy_(torch.as_tensor(noise, dtype=t.dtype, device=t.device))
+
+
+class Run:
+    """One precision form of the workload, stepped one xi step at a time."""
+
+    def __init__(self, precision: str, n_beam: int):
+        from lcodepy_amd.beam3d.beam_io import BeamDrain, BeamSource
+        from lcodepy_amd.config import Config
+        from lcodepy_amd.plasma3d.initialization import init_plasma
+        from lcodepy_amd.push_solver import PusherAndSolver3D
+        cfg = Config(workload_config(precision))
+        cfg._adjust_config_values_3d()
+        self.cfg = cfg
+        self.N = cfg.getint('window-width-steps')
+        fields, particles, currents, const, xi0 = init_plasma(cfg)
+        thermalize(particles)
+        self.n_plasma = sum(int(a.m.numel()) for a in particles.values())
+        self.solver = PusherAndSolver3D(cfg)
+        self.solver.verbose = False
+        beam = driver_beam(n_beam, cfg.getfloat('xi-step'))
+        self.it = self.solver.iter_dxi(fields, particles, currents, const, xi0,
+                                       BeamSource(cfg, beam), BeamDrain(cfg), 1.0)
+
+    def steps(self, k: int) -> None:
+        for _ in range(k):
+            next(self.it)
+
+    def ez(self) -> np.ndarray:
+        return np.array([float(v) for v in self.solver.ez_axis])
+
+
+def time_steps(run: Run, warmup: int, steps: int) -> float:
+    import torch
+    run.steps(warmup)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    run.steps(steps)
+    torch.cuda.synchronize()
+    return time.perf_counter() - t0
+
+
+def kernel_times(run: Run, launches: int) -> dict:
+    """Average duration of the push, the deposit and the field solve of the
+    live state, each launched alone between HIP events on the stream the
+    kernels use (torch's current stream)."""
+    import torch
+    solver = run.solver.plasma_solver
+    particles, fields, currents = run.solver._plasmastate
+    const_like = type('C', (), {})()
+    const_like.sorts = {'electrons': 0}
+    out = {}
+
+    def timed(fn):
+        fn()
+        torch.cuda.synchronize()
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record()
+        for _ in range(launches):
+            fn()
+        b.record()
+        torch.cuda.synchronize()
+        return a.elapsed_time(b) / launches * 1e-3
+
+    full = {s: a.copy() for s, a in particles.items()}
+    out['push'] = timed(lambda: solver.move_particles(fields, particles, full, const_like))
+    deposit = solver.compute_rhoj
+    const = type('C', (), {})()
+    const.sorts = {'electrons': 0}
+    const.ro_initial = currents.ro[1]
+    out['deposit'] = timed(lambda: deposit(particles, const))
+    rho = torch.zeros(run.N, run.N, dtype=torch.float64, device=fields.Ex.device)
+    out['fields'] = timed(lambda: solver.compute_fields(fields, fields, None, rho, rho,
+                                                        currents, currents))
+    return out
+
+
+def push_bytes(n: int, N: int, bulk: int) -> int:
+    """Algorithmic bytes of one smart-push launch (plasma3d/move.py:52-148):
+    reads x_init y_init x_offt_prev y_offt_prev x_offt y_offt (float64) and
+    px py pz (prev) q m (bulk type), writes x_offt y_offt (float64) and
+    px py pz (bulk type); plus one read of the six (N, N) fields."""
+    fbytes = 8 if bulk == 8 else 4
+    return n * (6 * 8 + 5 * bulk + 2 * 8 + 3 * bulk) + 6 * N * N * fbytes
+
+
+def cpu_baseline(sample_steps: int) -> dict:
+    """The oracle (float64 numpy + C, this repository's CPU restatement) on
+    the same workload: pushes on all host cores (OpenMP, as the reference's
+    Numba prange), deposit serial (as weights.py:35), scipy.fftpack DST/DCT."""
+    from lcodepy_amd.config import Config
+    from oracle import lcode_oracle as O
+    cfg = Config(workload_config('fp64'))
+    cfg._adjust_config_values_3d()
+    cores = os.cpu_count() or 1
+    p = O.Params.from_config(cfg, nthreads=cores)
+    fields, particles, currents, const, _ = O.init_plasma(p)
+    rng = np.random.default_rng(2)
+    for k in ('px', 'py', 'pz'):
+        particles['electrons'][k] = rng.normal(0.0, 1e-3, particles['electrons'][k].shape)
+    rho_prev = np.zeros((p.N, p.N))
+    grid = (np.arange(p.N) - p.N // 2) * p.h
+    rho = -np.exp(-(grid[:, None] ** 2 + grid[None, :] ** 2) / (2 * 0.25))
+    orig = O.compute_rhoj
+
+    def serial_rhoj(pp, particles_, const_, nthreads=None):
+        return orig(pp, particles_, const_, nthreads=1)
+
+    O.compute_rhoj = serial_rhoj
+    try:
+        t0 = time.perf_counter()
+        for _ in range(sample_steps):
+            particles, fields, currents = O.step_dxi(p, particles, fields, currents, const,
+                                                     rho, rho_prev)
+        dt = time.perf_counter() - t0
+    finally:
+        O.compute_rhoj = orig
+    model = ''
+    try:
+        with open('/proc/cpuinfo') as f:
+            model = next(l.split(':', 1)[1].strip() for l in f if l.startswith('model name'))
+    except (OSError, StopIteration):
+        pass
+    return {'value': sample_steps / dt, 'unit': 'xi-steps/s', 'cores': cores, 'kind': 'port',
+            'sample': f'{sample_steps} plasma xi steps of the same 513x513, 929,296-particle '
+                      f'float64 state (beam density fixed), deposit serial, pushes on {cores} '
+                      f'threads; {model}'}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument('--gpus', type=int, default=1)
+    ap.add_argument('--steps', type=int, default=100)
+    ap.add_argument('--warmup', type=int, default=10)
+    ap.add_argument('--beam-particles', type=int, default=1_000_000)
+    ap.add_argument('--kernel-launches', type=int, default=20)
+    ap.add_argument('--cpu-sample-steps', type=int, default=3)
+    ap.add_argument('--no-cpu-baseline', action='store_true')
+    ap.add_argument('--fp64-only', action='store_true')
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get('WORLD_SIZE', '1')) != 1:
+        print('bench.py measures one GPU: this build has no multi-GPU path '
+              '(independent runs would shard by run; see DESIGN.md)', file=sys.stderr)
+        return 2
+    import torch
+    torch.cuda.set_device(0)
+
+    results = {}
+    forms = ['fp64'] if args.fp64_only else ['fp32-bulk', 'fp64']
+    for form in forms:
+        print(f'[bench] {form}: building the workload', file=sys.stderr, flush=True)
+        run = Run(form, args.beam_particles)
+        elapsed = time_steps(run, args.warmup, args.steps)
+        run.solver.plasma_solver.ctx.raise_on_errors()
+        kt = kernel_times(run, args.kernel_launches)
+        results[form] = dict(elapsed=elapsed, ez=run.ez(), n=run.n_plasma, N=run.N, kt=kt)
+        print(f'[bench] {form}: {args.steps / elapsed:.1f} xi-steps/s, kernels {kt}',
+              file=sys.stderr, flush=True)
+        del run
+        torch.cuda.empty_cache()
+
+    head = forms[0]
+    r = results[head]
+    value = args.steps / r['elapsed']
+    bulk = 4 if head == 'fp32-bulk' else 8
+    pb = push_bytes(r['n'], r['N'], bulk)
+    t_push = r['kt']['push']
+    line = {
+        'metric': METRIC, 'value': value, 'unit': 'xi-steps/s', 'n_gpus': 1,
+        'steps': args.steps, 'warmup': args.warmup,
+        'ms_per_step': 1e3 * r['elapsed'] / args.steps, 'higher_is_better': True,
+        'scaling': 'weak', 'vs_baseline': None,
+        'dtype': 'fp32+fp64' if head == 'fp32-bulk' else 'fp64',
+        'data': 'synthetic (seeded plasma lattice with thermal momenta, seeded 1e6-particle '
+                'Gaussian driver)',
+        'config': {'workload': '513x513 window 25.6, 4 ppc (929,296 plasma macro-particles), '
+                               'xi_step 0.02, 1e6-particle driver n_b=1 sigma_r=0.5 sigma_xi=1',
+                   'precision': head, 'N': r['N'], 'plasma_particles': r['n'],
+                   'beam_particles': args.beam_particles},
+        'particle_pushes_per_s': 3 * r['n'] * value,
+        'kernel_avg_s': r['kt'],
+        'roofline': {'bound': 'hbm', 'kernel': 'k_push (one smart-push launch)',
+                     'achieved': pb / t_push / 1e9, 'peak': HBM_PEAK / 1e9, 'unit': 'GB/s',
+                     'frac': pb / t_push / HBM_PEAK, 'traffic': None},
+    }
+    if 'fp64' in results and head != 'fp64':
+        r64 = results['fp64']
+        v64 = args.steps / r64['elapsed']
+        n = min(len(r64['ez']), len(r['ez']))
+        ez_err = float(np.max(np.abs(r['ez'][:n] - r64['ez'][:n])) / np.max(np.abs(r64['ez'][:n])))
+        line['fp64_xi_steps_per_s'] = v64
+        line['fp32_bulk_over_fp64'] = value / v64
+        line['on_axis_ez_rel_err_vs_fp64'] = ez_err
+    if not args.no_cpu_baseline:
+        print('[bench] cpu baseline (oracle)', file=sys.stderr, flush=True)
+        line['cpu_baseline'] = cpu_baseline(args.cpu_sample_steps)
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+if __name__ == '__main__':
+    sys.exit(main())

@@ -1,0 +1,104 @@
+"""Beam layer calculators (beam3d/beam_calculator.py:60-181)."""
+from __future__ import annotations
+
+from .. import runtime
+from .._native import ptr, stream_ptr
+from ..config import Config
+from .data import BeamParticles
+from .move import get_move_beam_particles
+from .weights import get_deposit_beam
+
+
+class BeamCalculator:
+    def __init__(self, config: Config):
+        self.config = config
+        self.ctx = runtime.context(config)
+        self.grid_step_size = config.getfloat('transverse-step')
+        self.grid_steps = config.getint('window-width-steps')
+        self.time_step = config.getfloat('time-step')
+        self.substep_energy = config.getfloat('beam-substepping-energy')
+        self.deposit = get_deposit_beam(config)
+        self.move_particles = get_move_beam_particles(config)
+
+    def _zeros(self):
+        import torch
+        return torch.zeros(self.grid_steps, self.grid_steps, dtype=torch.float64,
+                           device=runtime.device())
+
+    def start_time_step(self):
+        self.rho_layout = self._zeros()
+
+    def layout_beam_layer(self, beam_layer: BeamParticles, plasma_layer_idx):
+        """Deposits the layer, returns the finished density of this layer and
+        keeps the part that belongs to the next one (beam_calculator.py:82-93)."""
+        rho_layout = self._zeros()
+        if beam_layer.size != 0:
+            self.deposit(plasma_layer_idx, beam_layer.x, beam_layer.y, beam_layer.xi,
+                         beam_layer.q_norm, self.rho_layout, rho_layout)
+        self.rho_layout, rho_layout = rho_layout, self.rho_layout
+        rho_layout /= self.grid_step_size ** 2
+        return rho_layout
+
+    def start_moving_layer(self, beam_layer: BeamParticles, fell_size):
+        """Substep multipliers for the particles that did not fall from the
+        previous layer (beam_calculator.py:97-114)."""
+        size = beam_layer.size - fell_size
+        if size <= 0:
+            return
+        self.ctx.check(self.ctx.lib.lc3d_beam_substep(
+            self.ctx.handle, size, ptr(beam_layer.q_m), ptr(beam_layer.pz),
+            float(self.substep_energy), float(self.time_step), ptr(beam_layer.dt),
+            ptr(beam_layer.remaining_steps), stream_ptr()), 'lc3d_beam_substep')
+
+    def move_beam_layer(self, beam_layer: BeamParticles, fell_size, pl_layer_idx,
+                        fields_after_layer, fields_before_layer):
+        import torch
+        n, dev = beam_layer.size, runtime.device()
+        lost = torch.zeros(n, dtype=torch.bool, device=dev)
+        moved = torch.zeros(n, dtype=torch.bool, device=dev)
+        fell = torch.zeros(n, dtype=torch.bool, device=dev)
+        if n != 0:
+            for k in ('q_m', 'dt', 'remaining_steps', 'id', 'x', 'y', 'xi', 'px', 'py', 'pz'):
+                t = getattr(beam_layer, k)
+                if not t.is_contiguous():
+                    setattr(beam_layer, k, t.contiguous())
+            self.start_moving_layer(beam_layer, fell_size)
+            self.move_particles(pl_layer_idx - 1, beam_layer, fields_after_layer,
+                                fields_before_layer, lost, moved, fell)
+        return beam_layer.get_layer(lost), beam_layer.get_layer(moved), beam_layer.get_layer(fell)
+
+    def create_next_layer(self, beam_layer_to_layout, fell_to_next_layer, ro_beam_full):
+        beam_layer_to_move = BeamParticles().append(beam_layer_to_layout).append(fell_to_next_layer)
+        return beam_layer_to_move, fell_to_next_layer.size, ro_beam_full.clone()
+
+
+class RigidBeamCalculator:
+    """A beam given as a charge-density callable f(xp, xi, x, y) on the
+    transverse grid, meshgrid 'xy' (beam_calculator.py:155-181)."""
+
+    def __init__(self, config: Config):
+        import torch
+        self.xi_step_size = config.getfloat('xi-step')
+        N = config.getint('window-width-steps')
+        h = config.getfloat('transverse-step')
+        grid = (torch.arange(N, dtype=torch.float64, device=runtime.device()) - N // 2) * h
+        self.x_grid, self.y_grid = torch.meshgrid(grid, grid, indexing='xy')
+
+    def start_time_step(self):
+        pass
+
+    def layout_beam_layer(self, beam_charge_distribution_function, plasma_layer_idx):
+        import numpy as np
+        import torch
+        xi = -plasma_layer_idx * self.xi_step_size
+        rho = beam_charge_distribution_function(torch, xi, self.x_grid, self.y_grid)
+        if isinstance(rho, np.ndarray):
+            rho = torch.as_tensor(rho, device=self.x_grid.device)
+        return torch.as_tensor(rho, dtype=torch.float64, device=self.x_grid.device).contiguous()
+
+    def move_beam_layer(self, beam_layer, fell_size, pl_layer_idx, fields_after_layer,
+                        fields_before_layer):
+        return None, None, None
+
+    def create_next_layer(self, beam_layer_to_layout, fell_to_next_layer, ro_beam_full):
+        return None, None, ro_beam_full.clone()

@@ -26,9 +26,9 @@ namespace lc3d {
 
 constexpr int WAVE = 64;
 constexpr int WAVES = 4;   // transforms per workgroup
-// Register arrays are sized by MAXM, the largest M a kernel instance serves
-// (256, 512 or 1024): MAXM / 64 packed samples and MAXM / 64 + 1 outputs per
-// lane.
+// Register arrays are sized by MAXM: the exact M for the lengths with a
+// compile-time plan (StaticPlan below), otherwise the largest M a runtime-plan
+// instance serves (64, 255, 511 or 1023).
 
 enum Field { F_EX = 0, F_EY, F_BX, F_BY, F_EZ, F_BZ, F_PHI };
 
@@ -144,9 +144,68 @@ __device__ __forceinline__ void fft_stage(cplx_t<T> *buf, const cplx_t<T> *__res
     __syncthreads();
 }
 
+// Radix plans fixed at compile time for the transform lengths of the
+// standard windows (N = 129, 257, 321, 385, 513, 769, 1025): every stage is
+// unrolled with constant strides.  Other lengths use the runtime plan.
+template <int M> struct StaticPlan { static constexpr int ns = 0; static constexpr int r[1] = {1}; };
+template <> struct StaticPlan<128> { static constexpr int ns = 3; static constexpr int r[3] = {8, 8, 2}; };
+template <> struct StaticPlan<256> { static constexpr int ns = 3; static constexpr int r[3] = {8, 8, 4}; };
+template <> struct StaticPlan<320> { static constexpr int ns = 3; static constexpr int r[3] = {8, 8, 5}; };
+template <> struct StaticPlan<384> { static constexpr int ns = 4; static constexpr int r[4] = {8, 8, 2, 3}; };
+template <> struct StaticPlan<512> { static constexpr int ns = 3; static constexpr int r[3] = {8, 8, 8}; };
+template <> struct StaticPlan<768> { static constexpr int ns = 4; static constexpr int r[4] = {8, 8, 4, 3}; };
+template <> struct StaticPlan<1024> { static constexpr int ns = 4; static constexpr int r[4] = {8, 8, 4, 4}; };
+
+template <typename T, int M, int R, int Ns>
+__device__ __forceinline__ void fft_stage_c(cplx_t<T> *buf, const cplx_t<T> *__restrict__ tw, int lane)
+{
+    using C = cplx_t<T>;
+    constexpr int nb = M / R;
+    constexpr int MAXB = (nb + WAVE - 1) / WAVE;
+    constexpr int stride = 2 * (M / (Ns * R));
+    C v[MAXB][R];
+#pragma unroll
+    for (int i = 0; i < MAXB; ++i) {
+        const int b = lane + i * WAVE;
+        if (b < nb) {
+#pragma unroll
+            for (int r = 0; r < R; ++r) v[i][r] = buf[b + r * nb];
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < MAXB; ++i) {
+        const int b = lane + i * WAVE;
+        if (b < nb) {
+            const int k = b % Ns;
+#pragma unroll
+            for (int r = 1; r < R; ++r) v[i][r] = cmul(v[i][r], tw[r * k * stride]);
+            Dft<T, R>::run(v[i], tw, M);
+            const int base = (b / Ns) * Ns * R + k;
+#pragma unroll
+            for (int r = 0; r < R; ++r) buf[base + r * Ns] = v[i][r];
+        }
+    }
+    __syncthreads();
+}
+
+template <typename T, int M, int S, int Ns>
+__device__ __forceinline__ void fft_static(cplx_t<T> *buf, const cplx_t<T> *__restrict__ tw, int lane)
+{
+    if constexpr (S < StaticPlan<M>::ns) {
+        constexpr int R = StaticPlan<M>::r[S];
+        fft_stage_c<T, M, R, Ns>(buf, tw, lane);
+        fft_static<T, M, S + 1, Ns * R>(buf, tw, lane);
+    }
+}
+
 template <typename T, int MAXM>
 __device__ void fft(cplx_t<T> *buf, const cplx_t<T> *__restrict__ tw, const FftPlan &pl, int lane)
 {
+    if constexpr (StaticPlan<MAXM>::ns > 0) {
+        fft_static<T, MAXM, 0, 1>(buf, tw, lane);   // MAXM is the exact length here
+        return;
+    }
     int Ns = 1;
     for (int s = 0; s < pl.ns; ++s) {
         const int R = pl.radix[s];
@@ -173,9 +232,9 @@ __device__ void r2r(bool dst, cplx_t<T> *buf, const cplx_t<T> *__restrict__ tw, 
                     int lane, bool active, Load load, Store store)
 {
     using C = cplx_t<T>;
-    constexpr int MAXN = MAXM / WAVE;
+    constexpr int MAXN = (MAXM + WAVE - 1) / WAVE;
     constexpr int MAXK = MAXM / WAVE + 1;
-    const int M = pl.M;
+    const int M = StaticPlan<MAXM>::ns > 0 ? MAXM : pl.M;
     // Pack the period-2M extension: z[n] = e[2n] + i e[2n+1].
     C z[MAXN];
 #pragma unroll
@@ -435,18 +494,41 @@ hipError_t launch_compute_fields(const SolveGeom &g, const SolveIn<TG, TPRED> &i
     const size_t lds = (size_t)WAVES * g.plan.M * sizeof(cplx_t<TG>);
     const dim3 rows((N + WAVES - 1) / WAVES, 7);
     const int M = g.plan.M;
-    if (M <= 256) {
-        k_rows_fwd<TG, TPRED, 256><<<rows, 256, lds, s>>>(g, in, rows.x);
-        k_cols<TG, 256><<<rows, 256, lds, s>>>(g);
-        k_rows_inv<TG, TFULL, THALF, 256><<<rows, 256, lds, s>>>(g, out, write_full);
-    } else if (M <= 512) {
-        k_rows_fwd<TG, TPRED, 512><<<rows, 256, lds, s>>>(g, in, rows.x);
-        k_cols<TG, 512><<<rows, 256, lds, s>>>(g);
-        k_rows_inv<TG, TFULL, THALF, 512><<<rows, 256, lds, s>>>(g, out, write_full);
+#define LC3D_SOLVE_STATIC(MM)                                                          \
+    case MM:                                                                          \
+        k_rows_fwd<TG, TPRED, MM><<<rows, 256, lds, s>>>(g, in, rows.x);              \
+        k_cols<TG, MM><<<rows, 256, lds, s>>>(g);                                      \
+        k_rows_inv<TG, TFULL, THALF, MM><<<rows, 256, lds, s>>>(g, out, write_full);  \
+        break;
+    bool done = true;
+    switch (M) {
+        LC3D_SOLVE_STATIC(128)
+        LC3D_SOLVE_STATIC(256)
+        LC3D_SOLVE_STATIC(320)
+        LC3D_SOLVE_STATIC(384)
+        LC3D_SOLVE_STATIC(512)
+        LC3D_SOLVE_STATIC(768)
+        LC3D_SOLVE_STATIC(1024)
+    default: done = false;
+    }
+#undef LC3D_SOLVE_STATIC
+    if (done) {
+    } else if (M <= 64) {
+        k_rows_fwd<TG, TPRED, 64><<<rows, 256, lds, s>>>(g, in, rows.x);
+        k_cols<TG, 64><<<rows, 256, lds, s>>>(g);
+        k_rows_inv<TG, TFULL, THALF, 64><<<rows, 256, lds, s>>>(g, out, write_full);
+    } else if (M <= 255) {
+        k_rows_fwd<TG, TPRED, 255><<<rows, 256, lds, s>>>(g, in, rows.x);
+        k_cols<TG, 255><<<rows, 256, lds, s>>>(g);
+        k_rows_inv<TG, TFULL, THALF, 255><<<rows, 256, lds, s>>>(g, out, write_full);
+    } else if (M <= 511) {
+        k_rows_fwd<TG, TPRED, 511><<<rows, 256, lds, s>>>(g, in, rows.x);
+        k_cols<TG, 511><<<rows, 256, lds, s>>>(g);
+        k_rows_inv<TG, TFULL, THALF, 511><<<rows, 256, lds, s>>>(g, out, write_full);
     } else {
-        k_rows_fwd<TG, TPRED, 1024><<<rows, 256, lds, s>>>(g, in, rows.x);
-        k_cols<TG, 1024><<<rows, 256, lds, s>>>(g);
-        k_rows_inv<TG, TFULL, THALF, 1024><<<rows, 256, lds, s>>>(g, out, write_full);
+        k_rows_fwd<TG, TPRED, 1023><<<rows, 256, lds, s>>>(g, in, rows.x);
+        k_cols<TG, 1023><<<rows, 256, lds, s>>>(g);
+        k_rows_inv<TG, TFULL, THALF, 1023><<<rows, 256, lds, s>>>(g, out, write_full);
     }
     const int fb = (int)std::min<int64_t>(1024, ((int64_t)N * N + 255) / 256);
     k_bz_finalize<TFULL, THALF><<<fb, 256, 0, s>>>(N, g.rowsum, out.prev[F_BZ], out.full[F_BZ],

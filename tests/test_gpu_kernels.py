@@ -136,7 +136,7 @@ def test_deposit_matches_oracle_and_is_deterministic(N):
     torch.cuda.synchronize()
     for u, v, w in zip(out1, out2, g):
         assert torch.equal(u, v)                       # bit-reproducible
-        assert relerr(host(u[0]), w) < 1e-13
+        assert relerr(host(u[0]), w) < 1e-12   # cell-order vs particle-order sums
         assert not torch.any(u[1])
 
 
@@ -170,7 +170,8 @@ def _gauss_beam(N, h, xi, amp=-0.05):
     return amp * np.exp(-r2 / 2) * np.exp(-(xi + 1.0) ** 2 / 2)
 
 
-@pytest.mark.parametrize('N,ion_model', [pytest.param(65, 'background', marks=FAULTED_N65),
+@pytest.mark.parametrize('N,ion_model', [(33, 'background'), (33, 'mobile'),
+                                         pytest.param(65, 'background', marks=FAULTED_N65),
                                          (129, 'background'), (129, 'mobile')])
 def test_plasma_step_matches_oracle(N, ion_model):
     from lcodepy_amd.plasma3d.initialization import init_plasma

@@ -1,0 +1,67 @@
+"""End-to-end runs of the HIP path against the reference's golden fixtures
+and against the oracle step by step.
+
+Tolerance (float64 form): the reference compares its own CPU runs at
+rtol 5e-8 (tests/integration/test_3D_part.py:7); the HIP path sums the
+deposits cell by cell and fuses multiply-adds, so after 201-505 xi steps it
+is compared as max |gpu - ref| / max |ref| <= 1e-6 per quantity (the
+on-axis Ez bound the float64 GPU form is held to)."""
+import os
+
+import numpy as np
+import pytest
+
+from oracle import lcode_oracle as O
+from tests.cases import adjusted, fixture_beam, fixture_config
+
+pytestmark = pytest.mark.gpu
+torch = pytest.importorskip('torch')
+
+
+def relerr(a, b):
+    return float(np.max(np.abs(a - b)) / np.max(np.abs(b)))
+
+
+@pytest.mark.parametrize('which', ['test1', 'beam_evol'])
+def test_simulation_matches_reference_fixture(which, golden_dir):
+    from lcodepy_amd.simulation import Simulation
+    print(f'fixture {which}', flush=True)
+    sim = Simulation(config=fixture_config(which), beam_parameters=fixture_beam(which),
+                     runas_filename='')
+    sim.push_solver.verbose = False
+    sim.step()
+    particles, fields, currents = sim.push_solver._plasmastate
+    el = particles['electrons']
+    golden = np.load(os.path.join(golden_dir, f'3D_{which}.npz'))
+    worst = {}
+    for k in ('x_offt', 'y_offt', 'px', 'py', 'pz'):
+        worst[k] = relerr(getattr(el, k).cpu().numpy()[::10, ::10], golden[k])
+    for k in ('Ex', 'Ey', 'Ez', 'Bx', 'By', 'Bz'):
+        worst[k] = relerr(getattr(fields, k).cpu().numpy()[::10, ::10], golden[k])
+    for k in ('ro', 'jx', 'jy', 'jz'):
+        worst[k] = relerr(getattr(currents, k).cpu().numpy()[:, ::10, ::10].sum(axis=0), golden[k])
+    print(which, {k: f'{v:.1e}' for k, v in worst.items()})
+    for k, v in worst.items():
+        assert v < 1e-6, (k, v)
+
+
+def test_on_axis_ez_matches_oracle_every_step():
+    """129 x 129 window, 4 ppc, a generated beam, 101 xi steps: the on-axis
+    Ez of every step within 1e-6 of the oracle's (normalised by max |Ez|)."""
+    from lcodepy_amd.beam_generator import generate_beam
+    from lcodepy_amd.simulation import Simulation
+    cfg = fixture_config('beam_evol')
+    cfg.update({'window-width': 3.2, 'transverse-step': 0.05, 'time-limit': 1})
+    beam_params = {'current': 0.5, 'particles_in_layer': 2000}
+    c = adjusted(cfg)
+    p = O.Params.from_config(c)
+    beam = generate_beam(c, beam_params)
+    _, _, _, _, ez_oracle = O.simulate(p, beam, 1)
+    sim = Simulation(config=cfg, beam_parameters=beam_params, runas_filename='')
+    sim.push_solver.verbose = False
+    sim.step()
+    ez_gpu = np.array([float(v) for v in sim.push_solver.ez_axis])
+    assert ez_gpu.shape == ez_oracle.shape
+    err = np.max(np.abs(ez_gpu - ez_oracle)) / np.max(np.abs(ez_oracle))
+    print(f'N={p.N}: on-axis Ez max error {err:.2e} over {len(ez_gpu)} xi steps')
+    assert err < 1e-6
