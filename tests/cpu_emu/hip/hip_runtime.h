@@ -1,0 +1,79 @@
+// CPU emulation of the part of HIP that lcodepy_amd's kernels use, for
+// checking them on the host under AddressSanitizer / UBSan before they run on
+// a GPU.  Each GPU thread of a block is a fiber (ucontext) on one OS thread;
+// blocks are spread over a small pool of OS threads; __syncthreads and the
+// wave-level exchange switch fibers and report a deadlock (a barrier that not
+// every thread reaches) instead of hanging.  Static LDS arrays become
+// thread_local function statics and dynamic LDS an exactly sized heap block,
+// so every out-of-range global or LDS access is an ASan report.  Device
+// allocations are host mallocs of the exact size.
+#pragma once
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <cstdlib>
+#include <functional>
+#include <thread>
+#include <vector>
+#include <atomic>
+#include <algorithm>
+#include <memory>
+#include <cstring>
+
+struct dim3 { unsigned x = 1, y = 1, z = 1; dim3(unsigned a = 1, unsigned b = 1, unsigned c = 1) : x(a), y(b), z(c) {} };
+struct double2 { double x, y; };
+struct float2 { float x, y; };
+typedef int hipError_t;
+typedef void *hipStream_t;
+#define hipSuccess 0
+inline hipError_t hipGetLastError() { return 0; }
+inline const char *hipGetErrorString(hipError_t) { return "emu"; }
+
+#define __global__
+#define __device__
+#define __host__
+#define __forceinline__ inline
+#define __launch_bounds__(x)
+#define __shared__ static thread_local
+#define __align__(n) alignas(n)
+#define __restrict__
+
+extern thread_local dim3 threadIdx, blockIdx;
+extern dim3 gridDim, blockDim;
+extern thread_local double emu_shfl_d[1024];
+void emu_sync_block();
+void emu_sync_wave();
+inline void __syncthreads() { emu_sync_block(); }
+inline double __shfl_xor(double v, int off) {
+    int t = threadIdx.x, w = t / 64;
+    emu_shfl_d[t] = v;
+    emu_sync_wave();
+    double r = emu_shfl_d[(t % 64 ^ off) + w * 64];
+    emu_sync_wave();
+    return r;
+}
+inline int atomicAdd(int *p, int v) { return __atomic_fetch_add(p, v, __ATOMIC_SEQ_CST); }
+inline int atomicSub(int *p, int v) { return __atomic_fetch_sub(p, v, __ATOMIC_SEQ_CST); }
+inline unsigned atomicOr(unsigned *p, unsigned v) { return __atomic_fetch_or(p, v, __ATOMIC_SEQ_CST); }
+inline double atomicAdd(double *p, double v) {
+    double old = *p, nw;
+    do { nw = old + v; } while (!__atomic_compare_exchange(p, &old, &nw, false, __ATOMIC_SEQ_CST, __ATOMIC_SEQ_CST));
+    return old;
+}
+using std::isfinite; using std::floor; using std::sqrt; using std::fmin; using std::fmax;
+using std::min; using std::max;
+
+// Runs body for every thread of every block of the grid.
+void emu_launch(dim3 grid, dim3 block, size_t lds, const std::function<void()> &body);
+template <typename... A> inline void emu_noop(A...) {}
+// Runtime stand-ins: host memory with exact sizes so ASan sees overruns.
+enum hipMemcpyKind { hipMemcpyHostToDevice, hipMemcpyDeviceToHost, hipMemcpyDeviceToDevice };
+template <typename T> inline hipError_t hipMalloc(T **p, size_t n) { *p = (T *)std::malloc(n ? n : 1); return 0; }
+inline hipError_t hipFree(void *p) { std::free(p); return 0; }
+inline hipError_t hipMemcpy(void *d, const void *s, size_t n, hipMemcpyKind) { std::memcpy(d, s, n); return 0; }
+inline hipError_t hipMemcpyAsync(void *d, const void *s, size_t n, hipMemcpyKind, hipStream_t) { std::memcpy(d, s, n); return 0; }
+inline hipError_t hipMemset(void *d, int v, size_t n) { std::memset(d, v, n); return 0; }
+inline hipError_t hipMemsetAsync(void *d, int v, size_t n, hipStream_t) { std::memset(d, v, n); return 0; }
+inline hipError_t hipDeviceSynchronize() { return 0; }
+inline hipError_t hipStreamSynchronize(hipStream_t) { return 0; }
+inline hipError_t hipGetDevice(int *d) { *d = 0; return 0; }

@@ -1,0 +1,57 @@
+"""The HIP kernels and the C ABI built for the CPU (tests/cpu_emu) and run
+under AddressSanitizer + UBSan: every global and LDS access of every entry
+point is bounds-checked on the host, and the GPU suite's own parity tests
+run against that build, before anything runs on a GPU."""
+import os
+import shutil
+import subprocess
+
+import pytest
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+EMU = os.path.join(HERE, 'cpu_emu')
+BUILD = os.path.join(EMU, '_build')
+
+
+def _libasan():
+    out = subprocess.run(['g++', '-print-file-name=libasan.so'], capture_output=True, text=True)
+    path = out.stdout.strip()
+    return path if os.path.isabs(path) and os.path.exists(path) else None
+
+
+@pytest.fixture(scope='module')
+def emu_build():
+    if shutil.which('g++') is None or _libasan() is None:
+        pytest.skip('needs g++ with AddressSanitizer')
+    subprocess.run(['make', '-C', EMU, '-j8', '-s'], check=True)
+    return BUILD
+
+
+def _env():
+    env = dict(os.environ)
+    env['ASAN_OPTIONS'] = 'detect_leaks=0:verify_asan_link_order=0:abort_on_error=1'
+    env['UBSAN_OPTIONS'] = 'halt_on_error=1:print_stacktrace=1'
+    return env
+
+
+def test_abi_entry_points_in_bounds(emu_build):
+    r = subprocess.run([os.path.join(emu_build, 'abi_check'), '33'], capture_output=True,
+                       text=True, env=_env(), timeout=300)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-4000:]
+    assert 'ALL OK' in r.stdout
+    assert 'ERROR: AddressSanitizer' not in r.stderr and 'runtime error' not in r.stderr
+
+
+def test_gpu_parity_suite_on_cpu_build(emu_build):
+    ids = ['tests/test_gpu_kernels.py::test_compute_fields_matches_oracle[33]',
+           'tests/test_gpu_kernels.py::test_compute_fields_matches_oracle[97]',
+           'tests/test_gpu_kernels.py::test_compute_fields_matches_oracle[129]',
+           'tests/test_gpu_kernels.py::test_deposit_matches_oracle_and_is_deterministic[65]',
+           'tests/test_gpu_kernels.py::test_push_matches_oracle[65]',
+           'tests/test_gpu_kernels.py::test_plasma_step_matches_oracle[33-mobile]']
+    env = _env()
+    env['LD_PRELOAD'] = _libasan()
+    r = subprocess.run(['python3', os.path.join(EMU, 'run_parity.py')] + ids,
+                       capture_output=True, text=True, env=env, timeout=600)
+    assert r.returncode == 0, r.stdout[-3000:] + r.stderr[-3000:]
+    assert 'ERROR: AddressSanitizer' not in r.stderr
