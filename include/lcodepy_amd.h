@@ -48,6 +48,11 @@ extern "C" {
 #define LC3D_FP64 0
 #define LC3D_FP32_BULK 1
 
+/* Field-solve implementations (lc3d_params.field_solver). */
+#define LC3D_SOLVER_MATRIX 0  /* DST-I/DCT-I as products with N x N transform matrices */
+#define LC3D_SOLVER_FFT 1     /* DST-I/DCT-I as LDS FFTs; only in builds that define
+                                 LC3D_FFT_SOLVER (the GPU library does not) */
+
 #define LC3D_OK 0
 #define LC3D_EINVAL 1    /* bad argument (null pointer, size beyond the context) */
 #define LC3D_EHIP 2      /* a HIP runtime call or kernel launch failed */
@@ -79,10 +84,11 @@ typedef struct lc3d_params {
      * the solver matrices (initialization.py:24, 43-44, 63); NULL = computed
      * here with the C library's sin */
     const double *eigenvalues;
+    int32_t field_solver;        /* LC3D_SOLVER_MATRIX or LC3D_SOLVER_FFT */
 } lc3d_params;
 
-/* Allocates every workspace (cell bins, spectral scratch, twiddles,
- * eigenvalues) on the current HIP device. */
+/* Allocates every workspace (cell bins, spectral scratch, transform
+ * matrices or twiddles, eigenvalues) on the current HIP device. */
 int lc3d_create(const lc3d_params *params, lc3d_ctx **out);
 void lc3d_destroy(lc3d_ctx *ctx);
 const char *lc3d_last_error(const lc3d_ctx *ctx);

@@ -17,6 +17,7 @@ LIB_PATH = os.path.join(_HERE, 'liblcodepy_amd.so')
 
 LC3D_FP64 = 0
 LC3D_FP32_BULK = 1
+SOLVERS = {'matrix': 0, 'fft': 1}
 
 ERR_BITS = {
     1: 'plasma particle stencil left the grid (not deposited)',
@@ -40,7 +41,8 @@ class Params(ctypes.Structure):
                 ('max_particles', ctypes.c_int64),
                 ('fft_nstages', ctypes.c_int32),
                 ('fft_radix', ctypes.c_int32 * 12),
-                ('eigenvalues', ctypes.POINTER(ctypes.c_double))]
+                ('eigenvalues', ctypes.POINTER(ctypes.c_double)),
+                ('field_solver', ctypes.c_int32)]
 
 
 class Fields6(ctypes.Structure):
@@ -138,7 +140,7 @@ class Context:
     def __init__(self, N: int, h: float, xi_step: float, boundary: float,
                  subtraction_coeff: float, lost_radius: float, precision: str,
                  max_particles: int, eigenvalues: np.ndarray | None = None,
-                 fft_radix: tuple | None = None):
+                 fft_radix: tuple | None = None, field_solver: str = 'matrix'):
         import torch
         if not torch.cuda.is_available():
             raise RuntimeError('lcodepy_amd needs a ROCm GPU (torch.cuda.is_available() is False)')
@@ -155,6 +157,9 @@ class Context:
         p.lost_radius = lost_radius
         p.precision = LC3D_FP32_BULK if precision == 'fp32-bulk' else LC3D_FP64
         p.max_particles = self.max_particles
+        if field_solver not in SOLVERS:
+            raise ValueError(f'unknown field solver {field_solver!r} (one of {sorted(SOLVERS)})')
+        p.field_solver = SOLVERS[field_solver]
         if fft_radix:
             p.fft_nstages = len(fft_radix)
             for i, r in enumerate(fft_radix):
@@ -166,7 +171,9 @@ class Context:
         h_ = _vp()
         rc = self.lib.lc3d_create(ctypes.byref(p), ctypes.byref(h_))
         if rc != 0:
-            raise RuntimeError(f'lc3d_create failed with code {rc} (N={N}, precision={precision})')
+            raise RuntimeError(f'lc3d_create failed with code {rc} (N={N}, precision={precision}, '
+                               f'field solver {field_solver}; the LDS-FFT solver is not built '
+                               'into the GPU library)')
         self.handle = h_
 
     def __del__(self):

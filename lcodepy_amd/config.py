@@ -67,6 +67,10 @@ DEFAULT_CONFIG_VALUES: dict[str, Any] = {
     #   'fp32-bulk' -- float32 push/deposit/transforms with float64 positions,
     #                  gamma - pz, 1/(1 - v_z) and predictor-corrector averages.
     'precision': 'fp64',
+    # Field-solve implementation of the HIP path: 'matrix' (DST-I/DCT-I as
+    # products with N x N transform matrices) or 'fft' (LDS FFTs; only in
+    # libraries built with it, not the GPU library).
+    'field-solver': 'matrix',
 }
 
 HIP_PU_TYPES = ('mi355x', 'gpu', 'hip')
@@ -191,6 +195,12 @@ class Config:
             raise ValueError(
                 f"processing-unit-type {pu!r} is not served by lcodepy_amd; "
                 f"use one of {HIP_PU_TYPES} (this package implements the HIP path only)")
+
+    def field_solver(self) -> str:
+        s = self.get('field-solver', 'matrix').lower()
+        if s not in ('matrix', 'fft'):
+            raise ValueError(f"field-solver must be 'matrix' or 'fft', got {s!r}")
+        return s
 
     def precision(self) -> str:
         p = self.get('precision', 'fp64').lower()

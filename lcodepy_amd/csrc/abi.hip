@@ -23,9 +23,11 @@ struct lc3d_ctx {
     int *count = nullptr, *start = nullptr, *blocks = nullptr, *key = nullptr, *idx = nullptr;
     unsigned *err = nullptr;
     // field solve
-    void *work = nullptr;
+    int solver = LC3D_SOLVER_MATRIX;
+    void *work = nullptr, *work2 = nullptr;
     double *rowsum = nullptr;
-    void *tw = nullptr;
+    void *tw = nullptr;                                      // FFT twiddles
+    void *tdst = nullptr, *tdct = nullptr, *tdct_t = nullptr;   // transform matrices
     double *lam = nullptr;
     std::string last_error;
 };
@@ -134,10 +136,31 @@ template <typename T> Fields6<T> f6(const lc3d_fields *f)
                       static_cast<const T *>(f->By), static_cast<const T *>(f->Bz)};
 }
 
+#ifdef LC3D_FFT_SOLVER
 SolveGeom solve_geom(const lc3d_ctx *c)
 {
     return SolveGeom{c->N, c->prm.grid_step_size, c->prm.xi_step_size, c->prm.subtraction_coeff,
                      c->plan, c->lam, c->tw, c->work, c->rowsum};
+}
+#endif
+
+MatrixGeom matrix_geom(const lc3d_ctx *c)
+{
+    return MatrixGeom{c->N, c->prm.grid_step_size, c->prm.xi_step_size, c->prm.subtraction_coeff,
+                      c->lam, c->tdst, c->tdct, c->tdct_t, c->work, c->work2, c->rowsum};
+}
+
+// Dispatches the solve to the context's implementation.
+template <typename TG, typename TPRED, typename TFULL, typename THALF>
+hipError_t run_solve(const lc3d_ctx *c, const SolveIn<TG, TPRED> &in,
+                     const SolveOut<TFULL, THALF> &out, bool write_full, hipStream_t st)
+{
+#ifdef LC3D_FFT_SOLVER
+    if (c->solver == LC3D_SOLVER_FFT)
+        return launch_compute_fields<TG, TPRED, TFULL, THALF>(solve_geom(c), in, out, write_full, st);
+#endif
+    return launch_compute_fields_matrix<TG, TPRED, TFULL, THALF>(matrix_geom(c), in, out,
+                                                                 write_full, st);
 }
 
 template <typename TG, typename TPRED>
@@ -252,20 +275,19 @@ int solve(lc3d_ctx *c, const lc3d_fields *pred, bool pred_f32, const lc3d_fields
           lc3d_fields *full, lc3d_fields *half, bool write_full, hipStream_t st)
 {
     hipError_t e;
-    const SolveGeom g = solve_geom(c);
     if (fp32(c)) {
         if (pred_f32)
-            e = launch_compute_fields<float, float, double, float>(
-                g, solve_in<float, float>(c, pred, rbF, rbP, cp, cf),
+            e = run_solve<float, float, double, float>(
+                c, solve_in<float, float>(c, pred, rbF, rbP, cp, cf),
                 solve_out<double, float>(prev, full, half), write_full, st);
         else
-            e = launch_compute_fields<float, double, double, float>(
-                g, solve_in<float, double>(c, pred, rbF, rbP, cp, cf),
+            e = run_solve<float, double, double, float>(
+                c, solve_in<float, double>(c, pred, rbF, rbP, cp, cf),
                 solve_out<double, float>(prev, full, half), write_full, st);
     } else {
         if (pred_f32) return fail(c, LC3D_EINVAL, "float32 predictor needs an FP32-bulk context");
-        e = launch_compute_fields<double, double, double, double>(
-            g, solve_in<double, double>(c, pred, rbF, rbP, cp, cf),
+        e = run_solve<double, double, double, double>(
+            c, solve_in<double, double>(c, pred, rbF, rbP, cp, cf),
             solve_out<double, double>(prev, full, half), write_full, st);
     }
     if (e != hipSuccess) return hip_fail(c, e, "compute_fields");
@@ -291,22 +313,35 @@ int lc3d_create(const lc3d_params *p, lc3d_ctx **out)
     c->N = N;
     c->cap = p->max_particles;
     const int M = N - 1;
-    if (p->fft_nstages > 0) {
-        c->plan.M = M;
-        c->plan.ns = p->fft_nstages;
-        for (int s = 0; s < 12; ++s) c->plan.radix[s] = s < p->fft_nstages ? p->fft_radix[s] : 0;
-        if (p->fft_nstages > 12 || !valid_plan(c->plan)) {
+    c->solver = p->field_solver;
+#ifdef LC3D_FFT_SOLVER
+    const bool fft = c->solver == LC3D_SOLVER_FFT;
+#else
+    const bool fft = false;
+#endif
+    if (c->solver != LC3D_SOLVER_MATRIX && !fft) {
+        delete c;
+        return LC3D_EINVAL;   // unknown solver, or the FFT solver in a build without it
+    }
+    if (fft) {
+        if (p->fft_nstages > 0) {
+            c->plan.M = M;
+            c->plan.ns = p->fft_nstages;
+            for (int s = 0; s < 12; ++s) c->plan.radix[s] = s < p->fft_nstages ? p->fft_radix[s] : 0;
+            if (p->fft_nstages > 12 || !valid_plan(c->plan)) {
+                delete c;
+                return LC3D_EINVAL;
+            }
+        } else if (!default_plan(M, c->plan)) {
             delete c;
             return LC3D_EINVAL;
         }
-    } else if (!default_plan(M, c->plan)) {
-        delete c;
-        return LC3D_EINVAL;
     }
     (void)hipGetDevice(&c->device);
     const size_t nn = (size_t)N * N;
     const size_t nblocks = (nn + 1 + 2047) / 2048;
-    const size_t tg = p->precision == LC3D_FP32_BULK ? sizeof(float) : sizeof(double);
+    const bool f32 = p->precision == LC3D_FP32_BULK;
+    const size_t tg = f32 ? sizeof(float) : sizeof(double);
     const size_t cap = (size_t)std::max<int64_t>(1, c->cap);
     bool ok = hipMalloc(&c->count, nn * sizeof(int)) == hipSuccess &&
               hipMalloc(&c->start, (nn + 1) * sizeof(int)) == hipSuccess &&
@@ -316,19 +351,19 @@ int lc3d_create(const lc3d_params *p, lc3d_ctx **out)
               hipMalloc(&c->err, sizeof(unsigned)) == hipSuccess &&
               hipMalloc(&c->work, 7 * nn * tg) == hipSuccess &&
               hipMalloc(&c->rowsum, N * sizeof(double)) == hipSuccess &&
-              hipMalloc(&c->tw, 2 * (size_t)M * 2 * tg) == hipSuccess &&
               hipMalloc(&c->lam, N * sizeof(double)) == hipSuccess;
+    if (ok && fft) ok = hipMalloc(&c->tw, 2 * (size_t)M * 2 * tg) == hipSuccess;
+    if (ok && !fft)
+        ok = hipMalloc(&c->work2, 7 * nn * tg) == hipSuccess &&
+             hipMalloc(&c->tdst, nn * tg) == hipSuccess &&
+             hipMalloc(&c->tdct, nn * tg) == hipSuccess &&
+             hipMalloc(&c->tdct_t, nn * tg) == hipSuccess;
     if (!ok) {
         lc3d_destroy(c);
         return LC3D_ENOMEM;
     }
-    // Twiddles exp(-i pi t / M), t = 0..2M-1, and the eigenvalues, in float64.
-    std::vector<double> twd(4 * (size_t)M), lam(N);
     const double pi = 3.14159265358979323846;
-    for (int t = 0; t < 2 * M; ++t) {
-        twd[2 * t] = std::cos(pi * t / M);
-        twd[2 * t + 1] = -std::sin(pi * t / M);
-    }
+    std::vector<double> lam(N);
     for (int k = 0; k < N; ++k) {
         if (p->eigenvalues) {
             lam[k] = p->eigenvalues[k];
@@ -337,12 +372,42 @@ int lc3d_create(const lc3d_params *p, lc3d_ctx **out)
             lam[k] = 4 / (p->grid_step_size * p->grid_step_size) * (s * s);
         }
     }
+    // Uploads a float64 host table in the solve's type.
+    auto upload = [&](void *dst, const std::vector<double> &v) {
+        if (f32) {
+            std::vector<float> w(v.begin(), v.end());
+            return hipMemcpy(dst, w.data(), w.size() * sizeof(float), hipMemcpyHostToDevice);
+        }
+        return hipMemcpy(dst, v.data(), v.size() * sizeof(double), hipMemcpyHostToDevice);
+    };
     hipError_t e = hipSuccess;
-    if (p->precision == LC3D_FP32_BULK) {
-        std::vector<float> twf(twd.begin(), twd.end());
-        e = hipMemcpy(c->tw, twf.data(), twf.size() * sizeof(float), hipMemcpyHostToDevice);
+    if (fft) {
+        // Twiddles exp(-i pi t / M), t = 0..2M-1.
+        std::vector<double> twd(4 * (size_t)M);
+        for (int t = 0; t < 2 * M; ++t) {
+            twd[2 * t] = std::cos(pi * t / M);
+            twd[2 * t + 1] = -std::sin(pi * t / M);
+        }
+        e = upload(c->tw, twd);
     } else {
-        e = hipMemcpy(c->tw, twd.data(), twd.size() * sizeof(double), hipMemcpyHostToDevice);
+        // S[k][n] = 2 sin(pi k n / M) (zero for k or n in {0, M}) and
+        // C[k][n] = w_n cos(pi k n / M), w_0 = w_M = 1, else 2: the
+        // unnormalised DST-I / DCT-I (fields.py:15-109) as N x N matrices.
+        // k n is reduced mod 2M so every angle is at most 2 pi.
+        std::vector<double> S(nn), C(nn), Ct(nn);
+        for (int k = 0; k < N; ++k) {
+            for (int n = 0; n < N; ++n) {
+                const int r = (int)(((int64_t)k * n) % (2 * M));
+                const double sn = (k == 0 || k == M || n == 0 || n == M) ? 0.0 : 2 * std::sin(pi * r / M);
+                const double cs = (n == 0 || n == M ? 1.0 : 2.0) * std::cos(pi * r / M);
+                S[(size_t)k * N + n] = sn;
+                C[(size_t)k * N + n] = cs;
+                Ct[(size_t)n * N + k] = cs;
+            }
+        }
+        e = upload(c->tdst, S);
+        if (e == hipSuccess) e = upload(c->tdct, C);
+        if (e == hipSuccess) e = upload(c->tdct_t, Ct);
     }
     if (e == hipSuccess) e = hipMemcpy(c->lam, lam.data(), N * sizeof(double), hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemset(c->count, 0, nn * sizeof(int));
@@ -366,8 +431,12 @@ void lc3d_destroy(lc3d_ctx *c)
     (void)hipFree(c->idx);
     (void)hipFree(c->err);
     (void)hipFree(c->work);
+    (void)hipFree(c->work2);
     (void)hipFree(c->rowsum);
     (void)hipFree(c->tw);
+    (void)hipFree(c->tdst);
+    (void)hipFree(c->tdct);
+    (void)hipFree(c->tdct_t);
     (void)hipFree(c->lam);
     delete c;
 }

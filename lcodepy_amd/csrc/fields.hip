@@ -19,8 +19,7 @@
 // split.  The complex FFT is a self-sorting Stockham FFT in LDS with radix
 // 2/4/8 stages plus generic 3/5/7/11/13 stages; one wavefront owns one
 // transform.  Twiddles exp(-i pi t / M) are computed on the host in float64.
-#include "common.h"
-#include "kernels.h"
+#include "solve_common.h"
 
 namespace lc3d {
 
@@ -29,13 +28,6 @@ constexpr int WAVES = 4;   // transforms per workgroup
 // Register arrays are sized by MAXM: the exact M for the lengths with a
 // compile-time plan (StaticPlan below), otherwise the largest M a runtime-plan
 // instance serves (64, 255, 511 or 1023).
-
-enum Field { F_EX = 0, F_EY, F_BX, F_BY, F_EZ, F_BZ, F_PHI };
-
-// Transform along y (rows) for each field: DST for fields that vanish at
-// y = edge, DCT otherwise; along x (columns) likewise.
-__device__ __forceinline__ bool dst_along_y(int f) { return f == F_EX || f == F_BY || f == F_EZ || f == F_PHI; }
-__device__ __forceinline__ bool dst_along_x(int f) { return f == F_EY || f == F_BX || f == F_EZ || f == F_PHI; }
 
 template <typename C> __device__ __forceinline__ C cmul(C a, C b) {
     C r;
@@ -289,59 +281,7 @@ __device__ void r2r(bool dst, cplx_t<T> *buf, const cplx_t<T> *__restrict__ tw, 
     __syncthreads();
 }
 
-// ---- pass A: right-hand sides (fields.py:112-266) + forward transform along y ----
-template <typename TG, typename TPRED> struct RhsEval {
-    const SolveIn<TG, TPRED> &in;
-    int N;
-    TG inv2h, invdxi, c;
-    __device__ TG roF(int o) const { return in.roF[0][o] + in.roF[1][o]; }
-    __device__ TG jxF(int o) const { return in.jxF[0][o] + in.jxF[1][o]; }
-    __device__ TG jyF(int o) const { return in.jyF[0][o] + in.jyF[1][o]; }
-    __device__ TG jzF(int o) const { return in.jzF[0][o] + in.jzF[1][o]; }
-    __device__ TG roP(int o) const { return in.roP[0][o] + in.roP[1][o]; }
-    __device__ TG jxP(int o) const { return in.jxP[0][o] + in.jxP[1][o]; }
-    __device__ TG jyP(int o) const { return in.jyP[0][o] + in.jyP[1][o]; }
-    __device__ TG jzP(int o) const { return in.jzP[0][o] + in.jzP[1][o]; }
-    __device__ TG ro_half(int o) const {
-        return (roF(o) + (TG)in.rbF[o] + roP(o) + (TG)in.rbP[o]) * TG(0.5);
-    }
-    __device__ TG jz_half(int o) const {
-        return (jzF(o) + (TG)in.rbF[o] + jzP(o) + (TG)in.rbP[o]) * TG(0.5);
-    }
-
-    __device__ TG operator()(int f, int i, int j) const {
-        const int o = i * N + j;
-        const bool inner = i > 0 && i < N - 1 && j > 0 && j < N - 1;
-        switch (f) {
-        case F_EX: {
-            const TG d = inner ? (ro_half(o + N) - ro_half(o - N)) * inv2h : TG(0);
-            return -((d - (jxP(o) - jxF(o)) * invdxi) - c * (TG)in.ExP[o]);
-        }
-        case F_EY: {
-            const TG d = inner ? (ro_half(o + 1) - ro_half(o - 1)) * inv2h : TG(0);
-            return -((d - (jyP(o) - jyF(o)) * invdxi) - c * (TG)in.EyP[o]);
-        }
-        case F_BX: {
-            const TG d = inner ? (jz_half(o + 1) - jz_half(o - 1)) * inv2h : TG(0);
-            return (d - (jyP(o) - jyF(o)) * invdxi) + c * (TG)in.BxP[o];
-        }
-        case F_BY: {
-            const TG d = inner ? (jz_half(o + N) - jz_half(o - N)) * inv2h : TG(0);
-            return -((d - (jxP(o) - jxF(o)) * invdxi) - c * (TG)in.ByP[o]);
-        }
-        case F_EZ:
-            return -((jxF(o + N) - jxF(o - N)) + (jyF(o + 1) - jyF(o - 1))) * inv2h;
-        case F_BZ: {
-            const TG a = inner ? jxF(o + 1) - jxF(o - 1) : TG(0);
-            const TG b = inner ? jyF(o + N) - jyF(o - N) : TG(0);
-            return -(a - b) * inv2h;
-        }
-        default:   // F_PHI
-            return roF(o) - jzF(o);
-        }
-    }
-};
-
+// ---- pass A: right-hand sides + forward transform along y -------------------
 __device__ __forceinline__ bool has_spectral_row(int f, int i, int N) {
     // Rows that carry data after the transform along y: all rows for the
     // fields whose x-transform is a DCT, the interior rows otherwise.
@@ -413,33 +353,8 @@ __global__ __launch_bounds__(256) void k_rows_inv(SolveGeom g, SolveOut<TFULL, T
     const bool spec = row_ok && has_spectral_row(f, i, N);
     const TG *src = reinterpret_cast<const TG *>(g.work) + (size_t)f * N * N + (size_t)i * N;
     const size_t ro = (size_t)i * N;
-    const TFULL *prev = out.prev[f];
-    TFULL *full = out.full[f];
-    THALF *half = out.half[f];
 
-    auto emit = [&](int jj, TG r) {
-        const size_t o = ro + jj;
-        switch (f) {
-        case F_EX: case F_EY: case F_BX: case F_BY: {
-            half[o] = (THALF)r;
-            if (write_full) full[o] = (TFULL)(2 * (TFULL)r - prev[o]);
-            break;
-        }
-        case F_EZ: {
-            const TFULL v = (TFULL)r;
-            if (write_full) full[o] = v;
-            half[o] = (THALF)((v + prev[o]) / 2);
-            break;
-        }
-        case F_PHI:
-            full[o] = (TFULL)r;   // fields_half.Phi is fields_full.Phi
-            half[o] = (THALF)r;
-            break;
-        default:                  // F_BZ: raw values, mean removed by k_bz_finalize
-            full[o] = (TFULL)r;
-            break;
-        }
-    };
+    auto emit = [&](int jj, TG r) { emit_output(out, f, ro + jj, r, write_full); };
 
     double s = 0;
     const bool ydst = dst_along_y(f);
@@ -459,30 +374,6 @@ __global__ __launch_bounds__(256) void k_rows_inv(SolveGeom g, SolveOut<TFULL, T
 #pragma unroll
         for (int off = WAVE / 2; off > 0; off >>= 1) s += __shfl_xor(s, off);
         if (lane == 0) g.rowsum[i] = s;
-    }
-}
-
-template <typename TFULL, typename THALF>
-__global__ __launch_bounds__(256) void k_bz_finalize(int N, const double *__restrict__ rowsum,
-                                                     const TFULL *__restrict__ prev,
-                                                     TFULL *__restrict__ full,
-                                                     THALF *__restrict__ half)
-{
-    __shared__ double red[256];
-    double s = 0;
-    for (int i = threadIdx.x; i < N; i += 256) s += rowsum[i];
-    red[threadIdx.x] = s;
-    __syncthreads();
-    for (int off = 128; off > 0; off >>= 1) {
-        if ((int)threadIdx.x < off) red[threadIdx.x] += red[threadIdx.x + off];
-        __syncthreads();
-    }
-    const double mean = red[0] / ((double)N * N);
-    const int64_t n = (int64_t)N * N;
-    for (int64_t o = blockIdx.x * (int64_t)256 + threadIdx.x; o < n; o += (int64_t)gridDim.x * 256) {
-        const TFULL v = (TFULL)((double)full[o] - mean);
-        full[o] = v;
-        half[o] = (THALF)((v + prev[o]) / 2);
     }
 }
 

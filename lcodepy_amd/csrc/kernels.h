@@ -98,6 +98,24 @@ hipError_t launch_compute_fields(const SolveGeom &g, const SolveIn<TG, TPRED> &i
                                  const SolveOut<TFULL, THALF> &out, bool write_full,
                                  hipStream_t s);
 
+// The same solve with each DST-I / DCT-I written as a product with its N x N
+// transform matrix (transform.hip).
+struct MatrixGeom {
+    int N;
+    double h, xi_step, c;   // c: subtraction coefficient
+    const double *lam;      // N eigenvalues 4/h^2 sin^2(k pi / 2(N-1))
+    const void *tdst;       // S[k][n] = 2 sin(pi k n / (N-1)), zero edge rows/columns (TG)
+    const void *tdct;       // C[k][n] = w_n cos(pi k n / (N-1)), w = 1 at n = 0, N-1, else 2
+    const void *tdct_t;     // C transposed
+    void *work1, *work2;    // 7 N*N TG each
+    double *rowsum;         // N
+};
+
+template <typename TG, typename TPRED, typename TFULL, typename THALF>
+hipError_t launch_compute_fields_matrix(const MatrixGeom &g, const SolveIn<TG, TPRED> &in,
+                                        const SolveOut<TFULL, THALF> &out, bool write_full,
+                                        hipStream_t s);
+
 // ---- beam (beam3d) -----------------------------------------------------------
 struct BeamGeom {
     int N;

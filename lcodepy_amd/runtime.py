@@ -79,12 +79,14 @@ def context(config: Config, max_particles: int | None = None) -> _native.Context
     radix = config.getraw('fft-radix-plan')
     radix = tuple(int(r) for r in radix) if radix else None
     dev = device().index
-    key = (dev, prec, radix) + tuple(sorted(g.items()))
+    solver = config.field_solver()
+    key = (dev, prec, radix, solver) + tuple(sorted(g.items()))
     with _LOCK:
         ctx = _CACHE.get(key)
         if ctx is None or ctx.max_particles < cap:
             with torch.cuda.device(dev):
                 ctx = _native.Context(eigenvalues=eigenvalues(g['N'], g['h']), precision=prec,
-                                      max_particles=cap, fft_radix=radix, **g)
+                                      max_particles=cap, fft_radix=radix,
+                                      field_solver=solver, **g)
             _CACHE[key] = ctx
     return ctx

@@ -1,13 +1,16 @@
 // Drives every entry point of the C ABI (include/lcodepy_amd.h) on the CPU
-// emulation, in both precision forms, with exactly sized host buffers:
-// field solve, deposit, push, fused plasma step (1 and 2 sorts) and the beam
-// kernels (including off-grid and non-finite beam particles).
+// emulation, in both precision forms and with both field solvers, with
+// exactly sized host buffers: field solve, deposit, push, fused plasma step
+// (1 and 2 sorts) and the beam kernels (including off-grid and non-finite
+// beam particles); then checks that the matrix and FFT solves agree.
 #include "hip/hip_runtime.h"
 #include "../../include/lcodepy_amd.h"
 #include <random>
 #include <vector>
 #include <cstring>
 #include <cassert>
+#include <cmath>
+#include <algorithm>
 
 static std::mt19937_64 rng(7);
 static double nrm() { static std::normal_distribution<double> d; return d(rng); }
@@ -22,7 +25,7 @@ template <typename T> T *arr(size_t n, double scale = 0.0) {
 struct Run {
     int N; double h = 0.1, dxi = 0.05; int prec; lc3d_ctx *ctx = nullptr; int64_t n = 0, side = 0;
     std::vector<double> grid;
-    Run(int N_, int prec_) : N(N_), prec(prec_) {
+    Run(int N_, int prec_, int solver) : N(N_), prec(prec_) {
         int steps = N - 30; side = steps / 2 * 2 * 2;   // fineness 2
         for (int i = 0; i < steps / 2 * 2; ++i) grid.push_back((0.5 + i) * h / 2);
         std::vector<double> g2; for (int i = (int)grid.size() - 1; i >= 0; --i) g2.push_back(-grid[i]);
@@ -30,6 +33,7 @@ struct Run {
         lc3d_params p{}; p.grid_steps = N; p.grid_step_size = h; p.xi_step_size = dxi;
         p.boundary = h * (N / 2.0 - 10); p.subtraction_coeff = 1.0;
         double R = h * N / 2; p.lost_radius = std::max(0.9 * R, R - 1); p.precision = prec; p.max_particles = n;
+        p.field_solver = solver;
         int rc = lc3d_create(&p, &ctx); if (rc) { printf("create failed %d\n", rc); exit(1); }
     }
 };
@@ -65,9 +69,9 @@ template <typename TG> lc3d_currents currents(int N, double scale) {
 
 #define CHECK(x) do { int rc_ = (x); if (rc_) { printf("FAIL %s -> %d\n", #x, rc_); exit(1); } } while (0)
 
-template <typename TP, typename TG, typename TH> void run_all(int N, int prec) {
-    printf("== N=%d prec=%d\n", N, prec); fflush(stdout);
-    Run r(N, prec); size_t nn = (size_t)N * N;
+template <typename TP, typename TG, typename TH> void run_all(int N, int prec, int solver) {
+    printf("== N=%d prec=%d solver=%d\n", N, prec, solver); fflush(stdout);
+    Run r(N, prec, solver); size_t nn = (size_t)N * N;
     lc3d_fields pred = fields<double>(nn, 0.01), prev = fields<double>(nn, 0.01), full = fields<double>(nn, 0), half = fields<TH>(nn, 0);
     lc3d_currents cp = currents<TG>(N, 0.1), cf = currents<TG>(N, 0.1);
     double *rbf = arr<double>(nn, 0.05), *rbp = arr<double>(nn, 0.05);
@@ -108,11 +112,45 @@ template <typename TP, typename TG, typename TH> void run_all(int N, int prec) {
     lc3d_destroy(r.ctx);
 }
 
+// The same solve through both implementations; returns max |a - b| / max |b|
+// over the seven full and half fields.
+template <typename TG, typename TH> double cross_check(int N, int prec) {
+    size_t nn = (size_t)N * N;
+    Run a(N, prec, LC3D_SOLVER_MATRIX), b(N, prec, LC3D_SOLVER_FFT);
+    lc3d_fields pred = fields<double>(nn, 0.01), prev = fields<double>(nn, 0.01);
+    lc3d_currents cp = currents<TG>(N, 0.1), cf = currents<TG>(N, 0.1);
+    double *rbf = arr<double>(nn, 0.05), *rbp = arr<double>(nn, 0.05);
+    lc3d_fields fa = fields<double>(nn, 0), ha = fields<TH>(nn, 0), fb = fields<double>(nn, 0), hb = fields<TH>(nn, 0);
+    CHECK(lc3d_compute_fields(a.ctx, &pred, 0, &prev, rbf, rbp, &cp, &cf, &fa, &ha, nullptr));
+    CHECK(lc3d_compute_fields(b.ctx, &pred, 0, &prev, rbf, rbp, &cp, &cf, &fb, &hb, nullptr));
+    double worst = 0;
+    auto cmp = [&](const void *x, const void *y, bool half) {
+        double d = 0, m = 0;
+        for (size_t k = 0; k < nn; ++k) {
+            double u = half ? (double)((const TH *)x)[k] : ((const double *)x)[k];
+            double v = half ? (double)((const TH *)y)[k] : ((const double *)y)[k];
+            d = std::max(d, std::fabs(u - v)); m = std::max(m, std::fabs(v));
+        }
+        worst = std::max(worst, m > 0 ? d / m : d);
+    };
+    const void *A[14] = {fa.Ex, fa.Ey, fa.Ez, fa.Bx, fa.By, fa.Bz, fa.Phi, ha.Ex, ha.Ey, ha.Ez, ha.Bx, ha.By, ha.Bz, ha.Phi};
+    const void *B[14] = {fb.Ex, fb.Ey, fb.Ez, fb.Bx, fb.By, fb.Bz, fb.Phi, hb.Ex, hb.Ey, hb.Ez, hb.Bx, hb.By, hb.Bz, hb.Phi};
+    for (int k = 0; k < 14; ++k) cmp(A[k], B[k], k >= 7);
+    lc3d_destroy(a.ctx); lc3d_destroy(b.ctx);
+    return worst;
+}
+
 int main(int argc, char **argv) {
     for (int i = 1; i < argc; ++i) {
         int N = atoi(argv[i]);
-        run_all<double, double, double>(N, LC3D_FP64);
-        run_all<float, float, float>(N, LC3D_FP32_BULK);
+        for (int solver : {LC3D_SOLVER_MATRIX, LC3D_SOLVER_FFT}) {
+            run_all<double, double, double>(N, LC3D_FP64, solver);
+            run_all<float, float, float>(N, LC3D_FP32_BULK, solver);
+        }
+        const double e64 = cross_check<double, double>(N, LC3D_FP64);
+        const double e32 = cross_check<float, float>(N, LC3D_FP32_BULK);
+        printf("matrix vs FFT solve N=%d: fp64 %.3e, fp32-bulk %.3e\n", N, e64, e32); fflush(stdout);
+        if (!(e64 < 1e-12) || !(e32 < 1e-4)) { printf("FAIL solver mismatch\n"); return 1; }
     }
     printf("ALL OK\n");
 }

@@ -66,14 +66,15 @@ def _sources(rng, N):
     return cur
 
 
-FAULTED_N65 = pytest.mark.skip(
-    reason='the N=65 field solve raised a GPU memory fault on its first run; no cause found in the '
-           'code (the CPU/ASan build of the same kernels is clean there), so the case stays off the '
-           'GPU until one is shown')
+@pytest.mark.skip(reason='the LDS-FFT field solve raised GPU memory faults at N=65 and N=129 '
+                         '(it passed at N=33); no cause was found in its code, IR or ISA and its '
+                         'CPU/ASan build is clean, so it is not built into the GPU library and '
+                         'its GPU tests stay off until a cause is shown')
+def test_fft_field_solver_on_gpu():
+    pass
 
 
-@pytest.mark.parametrize('N', [33, pytest.param(65, marks=FAULTED_N65), 97, 113, 129, 321, 353,
-                               417, 513])
+@pytest.mark.parametrize('N', [33, 65, 97, 113, 129, 321, 353, 417, 513])
 def test_compute_fields_matches_oracle(N):
     from lcodepy_amd.plasma3d.data import Arrays
     from lcodepy_amd.plasma3d.fields import get_field_computer
@@ -170,8 +171,7 @@ def _gauss_beam(N, h, xi, amp=-0.05):
     return amp * np.exp(-r2 / 2) * np.exp(-(xi + 1.0) ** 2 / 2)
 
 
-@pytest.mark.parametrize('N,ion_model', [(33, 'background'), (33, 'mobile'),
-                                         pytest.param(65, 'background', marks=FAULTED_N65),
+@pytest.mark.parametrize('N,ion_model', [(33, 'background'), (33, 'mobile'), (65, 'background'),
                                          (129, 'background'), (129, 'mobile')])
 def test_plasma_step_matches_oracle(N, ion_model):
     from lcodepy_amd.plasma3d.initialization import init_plasma

@@ -88,3 +88,17 @@ def test_ctypes_structures_match_the_c_layout(tmp_path):
     assert int(got['lc3d_fields']) == ctypes.sizeof(_native.Fields)
     assert int(got['lc3d_currents']) == ctypes.sizeof(_native.Currents)
     assert int(got['lc3d_fields6']) == ctypes.sizeof(_native.Fields6)
+
+
+@pytest.mark.skipif(not os.path.exists(_native.LIB_PATH), reason='library not built')
+def test_gpu_library_refuses_the_fft_solver_before_touching_the_device():
+    L = _native.load_library()
+    p = _native.Params()
+    p.grid_steps, p.grid_step_size, p.xi_step_size = 33, 0.1, 0.05
+    p.precision, p.max_particles = _native.LC3D_FP64, 100
+    p.field_solver = _native.SOLVERS['fft']
+    h = ctypes.c_void_p()
+    assert L.lc3d_create(ctypes.byref(p), ctypes.byref(h)) == 1   # LC3D_EINVAL
+    assert not h.value
+    p.field_solver = 7
+    assert L.lc3d_create(ctypes.byref(p), ctypes.byref(h)) == 1
