@@ -52,24 +52,58 @@ class BeamCalculator:
 
     def move_beam_layer(self, beam_layer: BeamParticles, fell_size, pl_layer_idx,
                         fields_after_layer, fields_before_layer):
+        """Moves the layer and splits it into (lost, moved, fell)
+        (beam_calculator.py:116-151)."""
+        return self.move_beam_layer_async(beam_layer, fell_size, pl_layer_idx,
+                                          fields_after_layer, fields_before_layer).result()
+
+    def move_beam_layer_async(self, beam_layer: BeamParticles, fell_size, pl_layer_idx,
+                              fields_after_layer, fields_before_layer) -> 'PendingMove':
+        """Launches the move and the order-preserving split of the layer on
+        the stream without waiting for them; ``result()`` of the returned
+        object waits for these kernels only (not for later work on the
+        stream) and gives (lost, moved, fell)."""
         import torch
         n, dev = beam_layer.size, runtime.device()
-        lost = torch.zeros(n, dtype=torch.bool, device=dev)
-        moved = torch.zeros(n, dtype=torch.bool, device=dev)
-        fell = torch.zeros(n, dtype=torch.bool, device=dev)
-        if n != 0:
-            for k in ('q_m', 'dt', 'remaining_steps', 'id', 'x', 'y', 'xi', 'px', 'py', 'pz'):
-                t = getattr(beam_layer, k)
-                if not t.is_contiguous():
-                    setattr(beam_layer, k, t.contiguous())
-            self.start_moving_layer(beam_layer, fell_size)
-            self.move_particles(pl_layer_idx - 1, beam_layer, fields_after_layer,
-                                fields_before_layer, lost, moved, fell)
-        return beam_layer.get_layer(lost), beam_layer.get_layer(moved), beam_layer.get_layer(fell)
+        if n == 0:
+            return PendingMove(None, None, None, None)
+        beam_layer.contiguous()
+        flags = torch.zeros(3, n, dtype=torch.bool, device=dev)   # lost, moved, fell
+        self.start_moving_layer(beam_layer, fell_size)
+        self.move_particles(pl_layer_idx - 1, beam_layer, fields_after_layer,
+                            fields_before_layer, flags[0], flags[1], flags[2])
+        # Selected particle k goes to column cumsum(flag)[k] - 1 of its
+        # group, the others to the spare column n (dropped by the slice).
+        dest = torch.where(flags, torch.cumsum(flags, dim=1) - 1, n)
+        out_f = beam_layer.f.new_empty(3, beam_layer.f.shape[0], n + 1)
+        out_i = beam_layer.i.new_empty(3, beam_layer.i.shape[0], n + 1)
+        for g in range(3):
+            out_f[g].index_copy_(1, dest[g], beam_layer.f)
+            out_i[g].index_copy_(1, dest[g], beam_layer.i)
+        counts = torch.empty(3, dtype=torch.int64, pin_memory=True)
+        counts.copy_(flags.sum(dim=1), non_blocking=True)
+        done = torch.cuda.Event()
+        done.record()
+        return PendingMove(out_f, out_i, counts, done)
 
     def create_next_layer(self, beam_layer_to_layout, fell_to_next_layer, ro_beam_full):
         beam_layer_to_move = BeamParticles().append(beam_layer_to_layout).append(fell_to_next_layer)
         return beam_layer_to_move, fell_to_next_layer.size, ro_beam_full.clone()
+
+
+class PendingMove:
+    """(lost, moved, fell) of a launched beam move, resolved on demand."""
+
+    def __init__(self, out_f, out_i, counts, event):
+        self.out_f, self.out_i, self.counts, self.event = out_f, out_i, counts, event
+
+    def result(self):
+        if self.out_f is None:
+            return BeamParticles(), BeamParticles(), BeamParticles()
+        self.event.synchronize()
+        sizes = self.counts.tolist()
+        return tuple(BeamParticles(f=self.out_f[g][:, :sizes[g]], i=self.out_i[g][:, :sizes[g]])
+                     for g in range(3))
 
 
 class RigidBeamCalculator:

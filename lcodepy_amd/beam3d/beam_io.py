@@ -19,30 +19,55 @@ class BeamSource:
             beam = beam_particles
         beam.xi_sorted()
         self.beam = beam
+        # -xi of the sorted beam in ascending order, on the host once, so
+        # that layer sizes need no device round trip.
+        self._neg_xi = -beam.xi.cpu().numpy()
+        self._offset = 0
 
     def get_beam_layer_to_layout(self, plasma_layer_idx):
         """Particles with xi_{i+1} <= xi < xi_i, counted over the whole
-        remaining beam as in beam_io.py:29-47."""
+        remaining beam and cut from its front, as in beam_io.py:29-47."""
         xi_min = -self.xi_step_size * plasma_layer_idx
         xi_max = -self.xi_step_size * (plasma_layer_idx + 1)
-        xi = self.beam.xi
-        n = int(((xi_max <= xi) & (xi < xi_min)).sum().item()) if xi.numel() else 0
-        layer, self.beam = self.beam.cut_beam_layer(n)
+        rest = self._neg_xi[self._offset:]
+        n = int(np.searchsorted(rest, -xi_max, side='right')
+                - np.searchsorted(rest, -xi_min, side='right'))
+        n = max(n, 0)
+        layer = self.beam.slice(self._offset, self._offset + n)
+        self._offset += n
         return layer
 
 
 class BeamDrain:
+    """Collects the moved layers; they are concatenated once, when read."""
+
     def __init__(self, config: Config):
-        self.beam_buffer = BeamParticles()
-        self.lost_buffer = BeamParticles()
+        self._layers, self._lost = [], []
+
+    @staticmethod
+    def _joined(layers):
+        out = BeamParticles()
+        for layer in layers:
+            out.append(layer)
+        if len(layers) > 1:
+            layers[:] = [out]
+        return out
+
+    @property
+    def beam_buffer(self) -> BeamParticles:
+        return self._joined(self._layers)
+
+    @property
+    def lost_buffer(self) -> BeamParticles:
+        return self._joined(self._lost)
 
     def push_beam_layer(self, beam_layer: BeamParticles):
         if beam_layer.size > 0:
-            self.beam_buffer.append(beam_layer)
+            self._layers.append(beam_layer)
 
     def push_beam_lost(self, lost_layer: BeamParticles):
         if lost_layer.size > 0:
-            self.lost_buffer.append(lost_layer)
+            self._lost.append(lost_layer)
 
     def beam_slice(self):
         return self.beam_buffer

@@ -1,8 +1,11 @@
 """Beam macro-particles as device tensors (beam3d/data.py:21-169).
 
-Structure of arrays: xi x y px py pz q_m q_norm dt (float64), id and
-remaining_steps (int64).  The on-disk / exchange record is the reference's
-``particle_dtype3d`` (beam3d/data.py:5-7)."""
+Structure of arrays, packed: the nine float64 quantities (xi x y px py pz q_m
+q_norm dt) are the rows of one (9, n) tensor and the two int64 ones (id,
+remaining_steps) the rows of one (2, n) tensor, so that cutting, gathering
+and appending a layer is one or two tensor operations rather than eleven.
+Each quantity is a contiguous row view (``beam.x`` ...).  The on-disk /
+exchange record is the reference's ``particle_dtype3d`` (beam3d/data.py:5-7)."""
 from __future__ import annotations
 
 import numpy as np
@@ -10,22 +13,34 @@ import numpy as np
 from ..beam_generator import particle_dtype3d
 
 FLOAT_KEYS = ('xi', 'x', 'y', 'px', 'py', 'pz', 'q_m', 'q_norm', 'dt')
-KEYS = ('xi', 'x', 'y', 'px', 'py', 'pz', 'q_m', 'q_norm', 'id', 'dt', 'remaining_steps')
+INT_KEYS = ('id', 'remaining_steps')
+KEYS = FLOAT_KEYS + INT_KEYS
+
+
+def _row_property(store: str, row: int):
+    def get(self):
+        return getattr(self, store)[row]
+
+    def set_(self, value):
+        getattr(self, store)[row].copy_(value)
+
+    return property(get, set_)
 
 
 class BeamParticles:
-    def __init__(self, xp=None):
+    def __init__(self, xp=None, f=None, i=None):
         import torch
         from ..runtime import device
         self.xp = torch
-        dev = device()
-        for k in KEYS:
-            dt = torch.float64 if k in FLOAT_KEYS else torch.int64
-            setattr(self, k, torch.zeros(0, dtype=dt, device=dev))
+        if f is None:
+            dev = device()
+            f = torch.zeros(len(FLOAT_KEYS), 0, dtype=torch.float64, device=dev)
+            i = torch.zeros(len(INT_KEYS), 0, dtype=torch.int64, device=dev)
+        self.f, self.i = f, i
 
     @property
     def size(self) -> int:
-        return int(self.id.numel())
+        return int(self.f.shape[1])
 
     @property
     def particles(self) -> np.ndarray:
@@ -38,11 +53,15 @@ class BeamParticles:
         import torch
         from ..runtime import device
         dev = device()
-        for k in particle_dtype3d.names:
-            dt = torch.int64 if k == 'id' else torch.float64
-            setattr(self, k, torch.as_tensor(np.ascontiguousarray(beam_array[k])).to(dev, dt))
-        self.dt = torch.zeros_like(self.q_norm)
-        self.remaining_steps = torch.zeros_like(self.id)
+        n = len(beam_array)
+        f = np.zeros((len(FLOAT_KEYS), n))
+        for r, k in enumerate(FLOAT_KEYS):
+            if k in particle_dtype3d.names:
+                f[r] = beam_array[k]
+        i = np.zeros((len(INT_KEYS), n), dtype=np.int64)
+        i[0] = beam_array['id']
+        self.f = torch.as_tensor(f).to(dev)
+        self.i = torch.as_tensor(i).to(dev)
 
     def load(self, *args, **kwargs):
         with np.load(*args, **kwargs) as loaded:
@@ -55,28 +74,45 @@ class BeamParticles:
         np.savez_compressed(*args, **kwargs,
                             **{k: getattr(self, k).cpu().numpy() for k in particle_dtype3d.names})
 
-    def _set(self, src: dict):
-        for k in KEYS:
-            setattr(self, k, src[k])
+    def contiguous(self) -> 'BeamParticles':
+        """Rows of a column slice are strided in the packed tensors; the
+        kernels take contiguous rows."""
+        if not self.f.is_contiguous():
+            self.f = self.f.contiguous()
+        if not self.i.is_contiguous():
+            self.i = self.i.contiguous()
+        return self
 
     def xi_sorted(self):
-        """Sort by descending xi (data.py:88-104)."""
+        """Sort by descending xi, stably (data.py:88-104)."""
         import torch
         idx = torch.argsort(-self.xi, stable=True)
-        self._set({k: getattr(self, k)[idx] for k in KEYS})
+        self.f, self.i = self.f[:, idx], self.i[:, idx]
 
     def get_layer(self, indexes_arr) -> 'BeamParticles':
-        out = BeamParticles()
-        out._set({k: getattr(self, k)[indexes_arr].contiguous() for k in KEYS})
-        return out
+        return BeamParticles(f=self.f[:, indexes_arr], i=self.i[:, indexes_arr])
+
+    def slice(self, start: int, stop: int) -> 'BeamParticles':
+        return BeamParticles(f=self.f[:, start:stop], i=self.i[:, start:stop])
 
     def cut_beam_layer(self, layer_length: int):
-        layer = BeamParticles()
-        layer._set({k: getattr(self, k)[:layer_length] for k in KEYS})
-        self._set({k: getattr(self, k)[layer_length:] for k in KEYS})
+        layer = self.slice(0, layer_length)
+        rest = self.slice(layer_length, self.size)
+        self.f, self.i = rest.f, rest.i
         return layer, self
 
     def append(self, other: 'BeamParticles') -> 'BeamParticles':
         import torch
-        self._set({k: torch.cat((getattr(self, k), getattr(other, k))) for k in KEYS})
+        if other.size:
+            if self.size:
+                self.f = torch.cat((self.f, other.f), dim=1)
+                self.i = torch.cat((self.i, other.i), dim=1)
+            else:
+                self.f, self.i = other.f, other.i
         return self
+
+
+for _r, _k in enumerate(FLOAT_KEYS):
+    setattr(BeamParticles, _k, _row_property('f', _r))
+for _r, _k in enumerate(INT_KEYS):
+    setattr(BeamParticles, _k, _row_property('i', _r))

@@ -37,15 +37,21 @@ class PusherAndSolver3D:
                  xi_plasma_layer_start, beam_source, beam_drain, current_time,
                  diagnostics_list=()):
         """The body of step_dt as a generator yielding the xi-step index after
-        every xi step (for callers that time or inspect single steps)."""
+        every xi step (for callers that time or inspect single steps).
+
+        Same order of dependencies as push_solver.py:215-286: the beam layer
+        moved at step k is layout(k-1) plus what fell from the move at step
+        k-1.  The split of a moved layer into (lost, moved, fell) is read back
+        one step later, after the plasma step of the next layer is queued, so
+        the host never waits for the stream to drain."""
         import torch
         from . import runtime
+        rigid = self.rigid_beam
         self.beam_calculator.start_time_step()
-        beam_layer_to_move = self.beam_particles_class()
-        fell_size = 0
         N = self.grid_steps
         ro_beam_prev = torch.zeros(N, N, dtype=torch.float64, device=runtime.device())
         self.ez_axis = []
+        pending, prev_layout, fell_size = None, None, 0
         start = round(-xi_plasma_layer_start / self.xi_step_size) + 1
         for xi_i in range(start, self.xi_steps + 1):
             beam_layer_to_layout = beam_source.get_beam_layer_to_layout(xi_i)
@@ -54,11 +60,19 @@ class PusherAndSolver3D:
             plasma_particles, plasma_fields, plasma_currents = self.plasma_solver.step_dxi(
                 plasma_particles, plasma_fields, plasma_currents, plasma_const_arrays,
                 ro_beam_full, ro_beam_prev)
-            lost, moved, fell = self.beam_calculator.move_beam_layer(
-                beam_layer_to_move, fell_size, xi_i, prev_plasma_fields, plasma_fields)
-            beam_layer_to_move, fell_size, ro_beam_prev = self.beam_calculator.create_next_layer(
-                beam_layer_to_layout, fell, ro_beam_full)
-            beam_drain.push_beam_layer(moved)
+            if not rigid:
+                beam_layer_to_move = self.beam_particles_class()
+                if pending is not None:
+                    _lost, moved, fell = pending.result()
+                    beam_drain.push_beam_layer(moved)
+                    beam_layer_to_move.append(prev_layout).append(fell)
+                    fell_size = fell.size
+                pending = self.beam_calculator.move_beam_layer_async(
+                    beam_layer_to_move, fell_size, xi_i, prev_plasma_fields, plasma_fields)
+                prev_layout = beam_layer_to_layout
+            # The beam calculator hands over a fresh density each step; a rigid
+            # beam's callable may return its own buffer.
+            ro_beam_prev = ro_beam_full.clone() if rigid else ro_beam_full
             xi_plasma_layer = -self.xi_step_size * xi_i
             for diagnostic in diagnostics_list:
                 diagnostic.after_step_dxi(current_time, xi_plasma_layer, plasma_particles,
@@ -69,6 +83,9 @@ class PusherAndSolver3D:
                       f'Ez={float(plasma_fields.Ez[N // 2, N // 2]):+.4e}', flush=True)
             self._plasmastate = (plasma_particles, plasma_fields, plasma_currents)
             yield xi_i
+        if pending is not None:
+            _lost, moved, _fell = pending.result()
+            beam_drain.push_beam_layer(moved)
         xi_plasma_layer = -self.xi_step_size * self.xi_steps
         for diagnostic in diagnostics_list:
             diagnostic.dump(current_time, xi_plasma_layer, plasma_particles, plasma_fields,

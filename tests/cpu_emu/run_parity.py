@@ -63,6 +63,26 @@ class _NoDevice:
 
 
 torch.cuda.device = _NoDevice
+
+
+class _DoneEvent:
+    def record(self, *a):
+        pass
+
+    def synchronize(self):
+        pass
+
+
+torch.cuda.Event = _DoneEvent
+_empty = torch.empty
+
+
+def _empty_unpinned(*a, **k):
+    k.pop('pin_memory', None)
+    return _empty(*a, **k)
+
+
+torch.empty = _empty_unpinned
 _to = torch.Tensor.to
 
 

@@ -65,3 +65,35 @@ def test_on_axis_ez_matches_oracle_every_step():
     err = np.max(np.abs(ez_gpu - ez_oracle)) / np.max(np.abs(ez_oracle))
     print(f'N={p.N}: on-axis Ez max error {err:.2e} over {len(ez_gpu)} xi steps')
     assert err < 1e-6
+
+
+def test_beam_evolution_matches_oracle():
+    """Three time steps of a 65 x 65 window with a generated beam: the beam
+    drained after the last time step (every particle's id, position and
+    momentum, matched by id) and the last step's on-axis Ez against the
+    oracle's run of the same three time steps."""
+    from lcodepy_amd.beam_generator import generate_beam
+    from lcodepy_amd.simulation import Simulation
+    cfg = fixture_config('beam_evol')
+    cfg.update({'window-width': 3.2, 'transverse-step': 0.1, 'window-length': 3,
+                'time-limit': 3, 'time-step': 1})
+    beam_params = {'current': 0.5, 'particles_in_layer': 300}
+    c = adjusted(cfg)
+    p = O.Params.from_config(c)
+    beam = generate_beam(c, beam_params)
+    _, _, _, beam_o, ez_o = O.simulate(p, beam, 3)
+    sim = Simulation(config=cfg, beam_parameters=beam_params, runas_filename='')
+    sim.push_solver.verbose = False
+    sim.step()
+    drained = sim.beam_drain.beam_slice()
+    ids = drained.id.cpu().numpy()
+    order_g = np.argsort(ids)
+    order_o = np.argsort(beam_o['id'])
+    print(f'N={p.N}: {len(ids)} drained particles (oracle {len(beam_o["id"])})')
+    assert np.array_equal(ids[order_g], beam_o['id'][order_o])
+    for k in ('x', 'y', 'xi', 'px', 'py', 'pz'):
+        g = getattr(drained, k).cpu().numpy()[order_g]
+        o = beam_o[k][order_o]
+        assert relerr(g, o) < 1e-9, k
+    ez_g = np.array([float(v) for v in sim.push_solver.ez_axis])
+    assert relerr(ez_g, ez_o) < 1e-6
