@@ -16,10 +16,12 @@
 //
 // Kernels, all on plain global buffers and static LDS:
 //   k_rhs       R of all seven fields (one thread per grid point);
-//   k_gemm_nn   C_f = A_f B_f for the seven fields of a launch (register-
-//               tiled, 64 x 64 tiles through LDS), optionally scaled by M;
+//   k_gemm_mfma C_f = A_f B_f for the seven fields of a launch (64 x 64
+//               tiles through LDS, 16x16x4 matrix-core products), optionally
+//               scaled by M;
 //   k_assemble  outputs (full / half fields) and the Bz row sums;
 //   k_bz_finalize  mean of Bz removed, half-step Bz.
+#include "mfma.h"
 #include "solve_common.h"
 
 namespace lc3d {
@@ -51,33 +53,41 @@ __global__ __launch_bounds__(256) void k_rhs(RhsEval<TG, TPRED> rhs, TG *__restr
 constexpr int GBM = 64, GBN = 64, GBK = 16;
 
 // C_f = A_f B_f for the field f = blockIdx.z; all matrices N x N row-major.
-// 256 threads: thread (tx, ty) = (tid % 16, tid / 16) owns rows ty + 16 r and
-// columns tx + 16 c (r, c < 4) of the block's 64 x 64 tile.  Tiles of A
-// (64 x 16, stored transposed) and B (16 x 64) pass through static LDS; loads
-// past the matrix edge read as zero and stores past it are skipped.
+// 256 threads = 4 waves; the block's 64 x 64 tile is split 2 x 2 between the
+// waves, each wave computing 32 x 32 as 2 x 2 16x16x4 matrix-core products per
+// step of 4 along k.  Tiles of A (64 x 16) and B (16 x 64) pass through
+// static LDS; loads past the matrix edge read as zero and stores past it are
+// skipped.  With SCALE the result is multiplied by the solver matrix of the
+// field at (row, column) = (kx, ky) (initialization.py:15-67).
 template <typename T, bool SCALE>
-__global__ __launch_bounds__(256) void k_gemm_nn(GemmBatch<T> bt, int N, ScaleGeom sg)
+__global__ __launch_bounds__(256) void k_gemm_mfma(GemmBatch<T> bt, int N, ScaleGeom sg)
 {
-    __shared__ T As[GBK][GBM + 1];
-    __shared__ T Bs[GBK][GBN];
+    using MM = Mfma16<T>;
+    using acc_t = typename MM::acc_t;
+    __shared__ T As[GBM][GBK + 1];
+    __shared__ T Bs[GBK][GBN + 1];
     const int f = blockIdx.z;
     const T *__restrict__ A = bt.A[f];
     const T *__restrict__ B = bt.B[f];
     T *__restrict__ C = bt.C[f];
     const int row0 = blockIdx.y * GBM, col0 = blockIdx.x * GBN;
-    const int tid = threadIdx.x, tx = tid % 16, ty = tid / 16;
-    T acc[4][4];
+    const int tid = threadIdx.x, lane = tid % 64, wave = tid / 64;
+    const int wr = (wave / 2) * 32, wc = (wave % 2) * 32;   // the wave's 32 x 32 in the tile
+    const int li = lane % 16, lk = lane / 16;
+    acc_t acc[2][2];
 #pragma unroll
-    for (int r = 0; r < 4; ++r)
+    for (int ti = 0; ti < 2; ++ti)
 #pragma unroll
-        for (int c = 0; c < 4; ++c) acc[r][c] = T(0);
+        for (int tj = 0; tj < 2; ++tj)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) acc[ti][tj][r] = T(0);
 
     for (int k0 = 0; k0 < N; k0 += GBK) {
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
-            const int r = ty + 16 * q;
-            const int gr = row0 + r, gk = k0 + tx;
-            As[tx][r] = (gr < N && gk < N) ? A[(size_t)gr * N + gk] : T(0);
+            const int r = tid / 16 + 16 * q, kk = tid % 16;
+            const int gr = row0 + r, gk = k0 + kk;
+            As[r][kk] = (gr < N && gk < N) ? A[(size_t)gr * N + gk] : T(0);
         }
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
@@ -87,16 +97,16 @@ __global__ __launch_bounds__(256) void k_gemm_nn(GemmBatch<T> bt, int N, ScaleGe
         }
         __syncthreads();
 #pragma unroll
-        for (int kk = 0; kk < GBK; ++kk) {
-            T a[4], b[4];
+        for (int k4 = 0; k4 < GBK; k4 += 4) {
+            T a[2], b[2];
 #pragma unroll
-            for (int r = 0; r < 4; ++r) a[r] = As[kk][ty + 16 * r];
+            for (int ti = 0; ti < 2; ++ti) a[ti] = As[wr + 16 * ti + li][k4 + lk];
 #pragma unroll
-            for (int c = 0; c < 4; ++c) b[c] = Bs[kk][tx + 16 * c];
+            for (int tj = 0; tj < 2; ++tj) b[tj] = Bs[k4 + lk][wc + 16 * tj + li];
 #pragma unroll
-            for (int r = 0; r < 4; ++r)
+            for (int ti = 0; ti < 2; ++ti)
 #pragma unroll
-                for (int c = 0; c < 4; ++c) acc[r][c] = fma(a[r], b[c], acc[r][c]);
+                for (int tj = 0; tj < 2; ++tj) acc[ti][tj] = MM::mma(a[ti], b[tj], acc[ti][tj]);
         }
         __syncthreads();
     }
@@ -105,24 +115,25 @@ __global__ __launch_bounds__(256) void k_gemm_nn(GemmBatch<T> bt, int N, ScaleGe
     const bool dx = dst_along_x(f), dy = dst_along_y(f);
     const double cc = (f == F_EZ || f == F_BZ || f == F_PHI) ? 0.0 : sg.c;
 #pragma unroll
-    for (int r = 0; r < 4; ++r) {
-        const int gr = row0 + ty + 16 * r;
+    for (int ti = 0; ti < 2; ++ti)
 #pragma unroll
-        for (int c = 0; c < 4; ++c) {
-            const int gc = col0 + tx + 16 * c;
-            if (gr < N && gc < N) {
-                T v = acc[r][c];
-                if (SCALE) {
-                    // Spectral indices (kx, ky) = (gr, gc).
-                    const bool ok = (!dx || (gr > 0 && gr < M)) && (!dy || (gc > 0 && gc < M)) &&
-                                    !(f == F_BZ && gr == 0 && gc == 0);
-                    const double mul = ok ? (1.0 / (sg.lam[gr] + sg.lam[gc] + cc)) / sg.d2 : 0.0;
-                    v = (T)((double)v * mul);
+        for (int tj = 0; tj < 2; ++tj)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int gr = row0 + wr + 16 * ti + MM::row(lk, r);
+                const int gc = col0 + wc + 16 * tj + li;
+                if (gr < N && gc < N) {
+                    T v = acc[ti][tj][r];
+                    if (SCALE) {
+                        // Spectral indices (kx, ky) = (gr, gc).
+                        const bool ok = (!dx || (gr > 0 && gr < M)) && (!dy || (gc > 0 && gc < M)) &&
+                                        !(f == F_BZ && gr == 0 && gc == 0);
+                        const double mul = ok ? (1.0 / (sg.lam[gr] + sg.lam[gc] + cc)) / sg.d2 : 0.0;
+                        v = (T)((double)v * mul);
+                    }
+                    C[(size_t)gr * N + gc] = v;
                 }
-                C[(size_t)gr * N + gc] = v;
             }
-        }
-    }
 }
 
 // One wave per (row, field): outputs of the row and, for Bz, its row sum.
@@ -176,10 +187,10 @@ hipError_t launch_compute_fields_matrix(const MatrixGeom &g, const SolveIn<TG, T
     }
     const dim3 tiles((N + GBN - 1) / GBN, (N + GBM - 1) / GBM, 7);
     k_rhs<TG, TPRED><<<dim3((unsigned)((nn + 255) / 256), 7), 256, 0, s>>>(rhs, w1);
-    k_gemm_nn<TG, false><<<tiles, 256, 0, s>>>(fy, N, sg);
-    k_gemm_nn<TG, true><<<tiles, 256, 0, s>>>(sx, N, sg);
-    k_gemm_nn<TG, false><<<tiles, 256, 0, s>>>(ix, N, sg);
-    k_gemm_nn<TG, false><<<tiles, 256, 0, s>>>(iy, N, sg);
+    k_gemm_mfma<TG, false><<<tiles, 256, 0, s>>>(fy, N, sg);
+    k_gemm_mfma<TG, true><<<tiles, 256, 0, s>>>(sx, N, sg);
+    k_gemm_mfma<TG, false><<<tiles, 256, 0, s>>>(ix, N, sg);
+    k_gemm_mfma<TG, false><<<tiles, 256, 0, s>>>(iy, N, sg);
     k_assemble<TG, TFULL, THALF><<<dim3((N + 3) / 4, 7), 256, 0, s>>>(w1, N, out, write_full,
                                                                        g.rowsum);
     const int fb = (int)std::min<int64_t>(1024, ((int64_t)N * N + 255) / 256);
