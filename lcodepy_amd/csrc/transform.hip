@@ -50,15 +50,19 @@ __global__ __launch_bounds__(256) void k_rhs(RhsEval<TG, TPRED> rhs, TG *__restr
     work[f * nn + o] = in_rhs_domain(f, i, j, N) ? rhs(f, i, j) : TG(0);
 }
 
-constexpr int GBM = 64, GBN = 64, GBK = 16;
+constexpr int GBM = 64, GBN = 64, GBK = 32;
+constexpr int GLA = GBM * GBK / 256;   // A elements per thread per tile
+constexpr int GLB = GBK * GBN / 256;   // B elements per thread per tile
 
 // C_f = A_f B_f for the field f = blockIdx.z; all matrices N x N row-major.
 // 256 threads = 4 waves; the block's 64 x 64 tile is split 2 x 2 between the
 // waves, each wave computing 32 x 32 as 2 x 2 16x16x4 matrix-core products per
-// step of 4 along k.  Tiles of A (64 x 16) and B (16 x 64) pass through
-// static LDS; loads past the matrix edge read as zero and stores past it are
-// skipped.  With SCALE the result is multiplied by the solver matrix of the
-// field at (row, column) = (kx, ky) (initialization.py:15-67).
+// step of 4 along k.  Tiles of A (64 x 32) and B (32 x 64) pass through
+// static LDS; the next tiles' global loads are issued into registers before
+// the current tiles are multiplied.  Loads past the matrix edge read as zero
+// and stores past it are skipped.  With SCALE the result is multiplied by the
+// solver matrix of the field at (row, column) = (kx, ky)
+// (initialization.py:15-67).
 template <typename T, bool SCALE>
 __global__ __launch_bounds__(256) void k_gemm_mfma(GemmBatch<T> bt, int N, ScaleGeom sg)
 {
@@ -74,6 +78,9 @@ __global__ __launch_bounds__(256) void k_gemm_mfma(GemmBatch<T> bt, int N, Scale
     const int tid = threadIdx.x, lane = tid % 64, wave = tid / 64;
     const int wr = (wave / 2) * 32, wc = (wave % 2) * 32;   // the wave's 32 x 32 in the tile
     const int li = lane % 16, lk = lane / 16;
+    // Loader coordinates: A rows ar + 8 q, column ak; B rows bk + 4 q, column bc.
+    const int ar = tid / GBK, ak = tid % GBK;
+    const int bk = tid / GBN, bc = tid % GBN;
     acc_t acc[2][2];
 #pragma unroll
     for (int ti = 0; ti < 2; ++ti)
@@ -82,20 +89,27 @@ __global__ __launch_bounds__(256) void k_gemm_mfma(GemmBatch<T> bt, int N, Scale
 #pragma unroll
             for (int r = 0; r < 4; ++r) acc[ti][tj][r] = T(0);
 
+    T ra[GLA], rb[GLB];
+    auto load = [&](int k0) {
+#pragma unroll
+        for (int q = 0; q < GLA; ++q) {
+            const int gr = row0 + ar + 8 * q, gk = k0 + ak;
+            ra[q] = (gr < N && gk < N) ? A[(size_t)gr * N + gk] : T(0);
+        }
+#pragma unroll
+        for (int q = 0; q < GLB; ++q) {
+            const int gk = k0 + bk + 4 * q, gc = col0 + bc;
+            rb[q] = (gk < N && gc < N) ? B[(size_t)gk * N + gc] : T(0);
+        }
+    };
+    load(0);
     for (int k0 = 0; k0 < N; k0 += GBK) {
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            const int r = tid / 16 + 16 * q, kk = tid % 16;
-            const int gr = row0 + r, gk = k0 + kk;
-            As[r][kk] = (gr < N && gk < N) ? A[(size_t)gr * N + gk] : T(0);
-        }
+        for (int q = 0; q < GLA; ++q) As[ar + 8 * q][ak] = ra[q];
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            const int kk = tid / 64 + 4 * q, c = tid % 64;
-            const int gk = k0 + kk, gc = col0 + c;
-            Bs[kk][c] = (gk < N && gc < N) ? B[(size_t)gk * N + gc] : T(0);
-        }
+        for (int q = 0; q < GLB; ++q) Bs[bk + 4 * q][bc] = rb[q];
         __syncthreads();
+        if (k0 + GBK < N) load(k0 + GBK);
 #pragma unroll
         for (int k4 = 0; k4 < GBK; k4 += 4) {
             T a[2], b[2];
