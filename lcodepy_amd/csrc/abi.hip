@@ -22,6 +22,7 @@ struct lc3d_ctx {
     // binning
     int *count = nullptr, *start = nullptr, *blocks = nullptr, *key = nullptr, *idx = nullptr;
     unsigned *err = nullptr;
+    void *rec = nullptr;
     // field solve
     int solver = LC3D_SOLVER_MATRIX;
     void *work = nullptr, *work2 = nullptr;
@@ -98,7 +99,8 @@ PushGeom push_geom(const lc3d_ctx *c, int64_t n)
 
 BinState bin_state(const lc3d_ctx *c)
 {
-    return BinState{c->count, c->start, c->blocks, c->key, c->idx, c->err};
+    return BinState{c->count, c->start, c->blocks, c->key, c->idx, c->err, c->rec,
+                    std::max<int64_t>(1, c->cap)};
 }
 
 template <typename TP> SortArrays<TP> sort_arrays(const lc3d_sort *s)
@@ -349,6 +351,7 @@ int lc3d_create(const lc3d_params *p, lc3d_ctx **out)
               hipMalloc(&c->key, cap * sizeof(int)) == hipSuccess &&
               hipMalloc(&c->idx, cap * sizeof(int)) == hipSuccess &&
               hipMalloc(&c->err, sizeof(unsigned)) == hipSuccess &&
+              hipMalloc(&c->rec, cap * deposit_record_bytes(f32)) == hipSuccess &&
               hipMalloc(&c->work, 7 * nn * tg) == hipSuccess &&
               hipMalloc(&c->rowsum, N * sizeof(double)) == hipSuccess &&
               hipMalloc(&c->lam, N * sizeof(double)) == hipSuccess;
@@ -430,6 +433,7 @@ void lc3d_destroy(lc3d_ctx *c)
     (void)hipFree(c->key);
     (void)hipFree(c->idx);
     (void)hipFree(c->err);
+    (void)hipFree(c->rec);
     (void)hipFree(c->work);
     (void)hipFree(c->work2);
     (void)hipFree(c->rowsum);

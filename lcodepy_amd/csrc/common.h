@@ -3,6 +3,8 @@
 #include <hip/hip_runtime.h>
 #include <cstdint>
 
+#include "kernels.h"
+
 namespace lc3d {
 
 // Sticky error bits the kernels raise; the host reads them back on demand.
@@ -46,6 +48,15 @@ template <typename T> __device__ __forceinline__ T w4_p2(T x) {
     T t = T(0.5) + x, t2 = t * t;
     return t2 * t2 * T(1.0 / 24.0);
 }
+// Tap I (0..4, offset I - 2) of weights4.
+template <int I, typename T> __device__ __forceinline__ T w4_tap(T x) {
+    if constexpr (I == 0) return w4_m2(x);
+    else if constexpr (I == 1) return w4_m1(x);
+    else if constexpr (I == 2) return w4_0(x);
+    else if constexpr (I == 3) return w4_p1(x);
+    else return w4_p2(x);
+}
+
 template <typename T> __device__ __forceinline__ void weights4(T x, T w[5]) {
     w[0] = w4_m2(x);
     w[1] = w4_m1(x);
@@ -85,6 +96,37 @@ __device__ __forceinline__ int xcd_swizzle(int b, int n) {
     const int n8 = n & ~7;
     if (b >= n8) return b;
     return (b & 7) * (n8 >> 3) + (b >> 3);
+}
+
+// Bins a plasma particle for the deposit: returns its cell key ix N + iy and
+// writes its record at rec[k], or returns -1 (with an error bit) when the 5x5
+// stencil would leave the grid or the position is not finite.  P Q Z M qv are
+// the stored px py pz m q.
+template <typename TP>
+__device__ __forceinline__ int bin_particle(double x, double y, double h, int N, double P, double Q,
+                                            double Z, double M, double qv, DepRec<TP> *rec,
+                                            int64_t k, unsigned *err)
+{
+    int ix, iy;
+    double xl, yl;
+    bool finite;
+    if (!stencil_cell(x, y, h, N, ix, iy, xl, yl, finite)) {
+        atomicOr(err, finite ? ERR_PARTICLE_OFF_GRID : ERR_NONFINITE);
+        return -1;
+    }
+    const double gm = sqrt(M * M + P * P + Q * Q + Z * Z);
+    const double dro = qv / (1 - Z / gm);
+    const double r_g = dro / gm;
+    DepRec<TP> r;
+    r.xl = (TP)xl;
+    r.yl = (TP)yl;
+    r.c0 = (TP)dro;
+    r.c1 = (TP)(P * r_g);
+    r.c2 = (TP)(Q * r_g);
+    r.c3 = (TP)(Z * r_g);
+    r.iy = iy;
+    rec[k] = r;
+    return ix * N + iy;
 }
 
 }  // namespace lc3d

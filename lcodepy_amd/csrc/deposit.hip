@@ -23,6 +23,8 @@
 #include "common.h"
 #include "kernels.h"
 
+#include <type_traits>
+
 namespace lc3d {
 
 constexpr int DEP_TILE = 16;
@@ -39,21 +41,18 @@ __global__ __launch_bounds__(256) void k_bin(int64_t n, double h, int N,
                                              const double *__restrict__ y_init,
                                              const double *__restrict__ xo,
                                              const double *__restrict__ yo,
-                                             const TP *__restrict__ q, BinState bs)
+                                             const TP *__restrict__ px, const TP *__restrict__ py,
+                                             const TP *__restrict__ pz, const TP *__restrict__ q,
+                                             const TP *__restrict__ m, BinState bs)
 {
     const int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (k >= n) return;
     int key = -1;
     if (q[k] != TP(0)) {
-        int ix, iy;
-        double xl, yl;
-        bool finite;
-        if (stencil_cell(x_init[k] + xo[k], y_init[k] + yo[k], h, N, ix, iy, xl, yl, finite)) {
-            key = ix * N + iy;
-            atomicAdd(&bs.count[key], 1);
-        } else {
-            atomicOr(bs.err, finite ? ERR_PARTICLE_OFF_GRID : ERR_NONFINITE);
-        }
+        key = bin_particle<TP>(x_init[k] + xo[k], y_init[k] + yo[k], h, N, (double)px[k],
+                               (double)py[k], (double)pz[k], (double)m[k], (double)q[k],
+                               static_cast<DepRec<TP> *>(bs.rec), k, bs.err);
+        if (key >= 0) atomicAdd(&bs.count[key], 1);
     }
     bs.key[k] = key;
 }
@@ -157,9 +156,10 @@ __global__ __launch_bounds__(256) void k_order_cells(int ncell, BinState bs)
 }
 
 // ---- gather deposit -------------------------------------------------------------------
-template <typename T> struct GatherCap;
-template <> struct GatherCap<double> { static constexpr int value = 512; };
-template <> struct GatherCap<float> { static constexpr int value = 1024; };
+// Particle slots per halo cell staged at once.
+template <typename T> struct SlotCap;
+template <> struct SlotCap<double> { static constexpr int value = 2; };
+template <> struct SlotCap<float> { static constexpr int value = 4; };
 
 template <typename T, bool KAHAN> struct Acc;
 template <typename T> struct Acc<T, false> {
@@ -178,114 +178,114 @@ template <typename T> struct Acc<T, true> {
     __device__ __forceinline__ T get() const { return s; }
 };
 
+// One block per 16 x 16 tile of output cells, one thread per output cell.
+// The records (written by the pass that binned each particle) of the 20 x 20
+// halo cells' particles are staged into LDS slot by slot
+// ([slot][halo row][halo column]), so the 64 lanes of a wave read 16
+// consecutive cells of 4 rows and all run the same 25 x slots loop.  Each
+// output cell sums its 5 x 5 source cells row by row (x ascending), then by
+// column, then in particle-id order within a cell; passes over slots K at a
+// time interleave this order when a halo cell holds more than K particles.
+// No floating-point atomics: the result is reproducible bit for bit.
 template <typename TP, typename TG, bool KAHAN>
 __global__ __launch_bounds__(256) void k_deposit_gather(
-    int N, double h, int tiles, const int *__restrict__ start, const int *__restrict__ idx,
-    const double *__restrict__ x_init, const double *__restrict__ y_init,
-    const double *__restrict__ xo, const double *__restrict__ yo, const TP *__restrict__ px,
-    const TP *__restrict__ py, const TP *__restrict__ pz, const TP *__restrict__ q,
-    const TP *__restrict__ m, TG *__restrict__ ro, TG *__restrict__ jx, TG *__restrict__ jy,
-    TG *__restrict__ jz, const TG *__restrict__ bg_ro, TG *__restrict__ ro1,
-    TG *__restrict__ jx1, TG *__restrict__ jy1, TG *__restrict__ jz1)
+    int N, double h, const int *__restrict__ start, BinState bs, TG *__restrict__ ro,
+    TG *__restrict__ jx, TG *__restrict__ jy, TG *__restrict__ jz, const TG *__restrict__ bg_ro,
+    TG *__restrict__ ro1, TG *__restrict__ jx1, TG *__restrict__ jy1, TG *__restrict__ jz1)
 {
     using T = TP;   // arithmetic type of the tile
-    constexpr int CAP = GatherCap<T>::value;
-    __shared__ int hs[DEP_HALO][DEP_HALO + 1];   // global list position of each halo cell
-    __shared__ int rowoff[DEP_HALO + 1];         // halo-list offset of each halo row
-    __shared__ T s_wx[5][CAP];
-    __shared__ T s_wy[5][CAP];
-    __shared__ T s_c[4][CAP];
-    __shared__ int s_cy[CAP];
+    constexpr int K = SlotCap<T>::value;
+    constexpr int H = DEP_HALO;
+    __shared__ int s_start[H][H];
+    __shared__ int s_cnt[H][H];
+    __shared__ T s_xl[K][H][H];
+    __shared__ T s_yl[K][H][H];
+    __shared__ T s_c[4][K][H][H];
+    __shared__ int s_max;
 
     const int tile = xcd_swizzle(blockIdx.x, gridDim.x);
     const int tpr = (N + DEP_TILE - 1) / DEP_TILE;
     const int X0 = (tile / tpr) * DEP_TILE, Y0 = (tile % tpr) * DEP_TILE;
-    (void)tiles;
-    const int hx0 = max(0, X0 - 2), hx1 = min(N, X0 + DEP_TILE + 2);
-    const int hy0 = max(0, Y0 - 2), hy1 = min(N, Y0 + DEP_TILE + 2);
-    const int nrow = hx1 - hx0, ncol = hy1 - hy0;
+    const DepRec<T> *__restrict__ rec = static_cast<const DepRec<T> *>(bs.rec);
     const int t = threadIdx.x;
-
-    for (int i = t; i < nrow * (ncol + 1); i += 256) {
-        const int r = i / (ncol + 1), c = i % (ncol + 1);
-        hs[r][c] = start[(hx0 + r) * N + hy0 + c];
-    }
+    if (t == 0) s_max = 0;
     __syncthreads();
-    if (t == 0) {
-        int acc = 0;
-        for (int r = 0; r < nrow; ++r) {
-            rowoff[r] = acc;
-            acc += hs[r][ncol] - hs[r][0];
+    int my_max = 0;
+    for (int i = t; i < H * H; i += 256) {
+        const int r = i / H, c = i % H;
+        const int gx = X0 - 2 + r, gy = Y0 - 2 + c;
+        int st = 0, cnt = 0;
+        if (gx >= 0 && gx < N && gy >= 0 && gy < N) {
+            const int cell = gx * N + gy;
+            st = start[cell];
+            cnt = start[cell + 1] - st;
         }
-        rowoff[nrow] = acc;
+        s_start[r][c] = st;
+        s_cnt[r][c] = cnt;
+        my_max = max(my_max, cnt);
     }
+    atomicMax(&s_max, my_max);
     __syncthreads();
-    const int total = rowoff[nrow];
+    const int maxcnt = s_max;
 
-    // This thread's output cell and, per source row, its particle range.
     const int tx = t / DEP_TILE, ty = t % DEP_TILE;
     const int ox = X0 + tx, oy = Y0 + ty;
-    const bool valid = ox < N && oy < N;
-    int lo[5], hi[5];
-#pragma unroll
-    for (int kr = 0; kr < 5; ++kr) {
-        const int r = ox - 2 + kr - hx0;
-        lo[kr] = hi[kr] = 0;
-        if (valid && r >= 0 && r < nrow) {
-            const int c0 = max(oy - 2, hy0) - hy0, c1 = min(oy + 3, hy1) - hy0;
-            lo[kr] = rowoff[r] + hs[r][c0] - hs[r][0];
-            hi[kr] = rowoff[r] + hs[r][c1] - hs[r][0];
-        }
-    }
-
     Acc<T, KAHAN> a_ro, a_jx, a_jy, a_jz;
     const T inv_h = T(1.0 / h);
-    for (int k0 = 0; k0 < total; k0 += CAP) {
-        const int k1 = min(total, k0 + CAP);
-        // Stage: quartic weights (pre-divided by h) and charges per particle.
-        for (int j = k0 + t; j < k1; j += 256) {
-            int r = 0;
-            while (rowoff[r + 1] <= j) ++r;
-            const int p = idx[hs[r][0] + (j - rowoff[r])];
-            int ix, iy;
-            double xl, yl;
-            bool finite;
-            stencil_cell(x_init[p] + xo[p], y_init[p] + yo[p], h, N, ix, iy, xl, yl, finite);
-            T w[5];
-            weights4<T>((T)xl, w);
-#pragma unroll
-            for (int i = 0; i < 5; ++i) s_wx[i][j - k0] = w[i] * inv_h;
-            weights4<T>((T)yl, w);
-#pragma unroll
-            for (int i = 0; i < 5; ++i) s_wy[i][j - k0] = w[i] * inv_h;
-            const double P = px[p], Q = py[p], R = pz[p], M = m[p];
-            const double gm = sqrt(M * M + P * P + Q * Q + R * R);
-            const double dro = (double)q[p] / (1 - R / gm);
-            const double r_g = dro / gm;
-            s_c[0][j - k0] = (T)dro;
-            s_c[1][j - k0] = (T)(P * r_g);
-            s_c[2][j - k0] = (T)(Q * r_g);
-            s_c[3][j - k0] = (T)(R * r_g);
-            s_cy[j - k0] = iy;
+    for (int k0 = 0; k0 < maxcnt; k0 += K) {
+        for (int e = t; e < K * H * H; e += 256) {
+            const int k = e / (H * H), cell = e % (H * H);
+            const int r = cell / H, c = cell % H;
+            T xl = 0, yl = 0, c0 = 0, c1 = 0, c2 = 0, c3 = 0;
+            if (k0 + k < s_cnt[r][c]) {
+                const DepRec<T> d = rec[bs.idx[s_start[r][c] + k0 + k]];
+                xl = d.xl;
+                yl = d.yl;
+                c0 = d.c0;
+                c1 = d.c1;
+                c2 = d.c2;
+                c3 = d.c3;
+            }
+            s_xl[k][r][c] = xl;
+            s_yl[k][r][c] = yl;
+            s_c[0][k][r][c] = c0;
+            s_c[1][k][r][c] = c1;
+            s_c[2][k][r][c] = c2;
+            s_c[3][k][r][c] = c3;
         }
         __syncthreads();
-        // Accumulate source rows in ascending order: kx = ox - sx = 2 - kr.
-#pragma unroll
-        for (int kr = 0; kr < 5; ++kr) {
-            const int b = max(lo[kr], k0), e = min(hi[kr], k1);
-            for (int j = b; j < e; ++j) {
-                const int jj = j - k0;
-                const T w = s_wx[4 - kr][jj] * s_wy[oy - s_cy[jj] + 2][jj];
-                a_ro.add(s_c[0][jj] * w);
-                a_jx.add(s_c[1][jj] * w);
-                a_jy.add(s_c[2][jj] * w);
-                a_jz.add(s_c[3][jj] * w);
-            }
-        }
+        // Source cell (tx + sx, ty + sy) in halo coordinates lies at offset
+        // (2 - sx, 2 - sy) from this output cell: taps 4 - sx and 4 - sy.
+        const int kn = min(K, maxcnt - k0);
+        auto row = [&](auto sxc) {
+            constexpr int sx = decltype(sxc)::value;
+            auto col = [&](auto syc) {
+                constexpr int sy = decltype(syc)::value;
+                for (int k = 0; k < kn; ++k) {
+                    const T wx = w4_tap<4 - sx>(s_xl[k][tx + sx][ty + sy]) * inv_h;
+                    const T wy = w4_tap<4 - sy>(s_yl[k][tx + sx][ty + sy]) * inv_h;
+                    const T w = wx * wy;
+                    a_ro.add(s_c[0][k][tx + sx][ty + sy] * w);
+                    a_jx.add(s_c[1][k][tx + sx][ty + sy] * w);
+                    a_jy.add(s_c[2][k][tx + sx][ty + sy] * w);
+                    a_jz.add(s_c[3][k][tx + sx][ty + sy] * w);
+                }
+            };
+            col(std::integral_constant<int, 0>{});
+            col(std::integral_constant<int, 1>{});
+            col(std::integral_constant<int, 2>{});
+            col(std::integral_constant<int, 3>{});
+            col(std::integral_constant<int, 4>{});
+        };
+        row(std::integral_constant<int, 0>{});
+        row(std::integral_constant<int, 1>{});
+        row(std::integral_constant<int, 2>{});
+        row(std::integral_constant<int, 3>{});
+        row(std::integral_constant<int, 4>{});
         __syncthreads();
     }
 
-    if (valid) {
+    if (ox < N && oy < N) {
         const int o = ox * N + oy;
         ro[o] = (TG)a_ro.get();
         jx[o] = (TG)a_jx.get();
@@ -316,7 +316,8 @@ hipError_t launch_deposit(const PushGeom &g, const SortArrays<TP> &a, bool use_f
     const int pblocks = (int)((g.n + 255) / 256);
     if (g.n > 0) {
         if (!binned)
-            k_bin<TP><<<pblocks, 256, 0, s>>>(g.n, g.h, N, a.x_init, a.y_init, xo, yo, q, bs);
+            k_bin<TP><<<pblocks, 256, 0, s>>>(g.n, g.h, N, a.x_init, a.y_init, xo, yo, px, py, pz,
+                                              q, a.m, bs);
         const int nb = (ncell + 1 + SCAN_TILE - 1) / SCAN_TILE;
         k_scan_blocks<<<nb, SCAN_THREADS, 0, s>>>(ncell, bs.count, bs.blocks);
         k_scan_final<<<nb, SCAN_THREADS, 0, s>>>(ncell, nb, bs.count, bs.blocks, bs.start);
@@ -331,9 +332,8 @@ hipError_t launch_deposit(const PushGeom &g, const SortArrays<TP> &a, bool use_f
     const int tpr = (N + DEP_TILE - 1) / DEP_TILE;
     const int tiles = tpr * tpr;
     constexpr bool kahan = sizeof(TP) == 4;
-    k_deposit_gather<TP, TG, kahan><<<tiles, 256, 0, s>>>(
-        N, g.h, tiles, bs.start, bs.idx, a.x_init, a.y_init, xo, yo, px, py, pz, q, a.m, ro, jx,
-        jy, jz, bg_ro, ro1, jx1, jy1, jz1);
+    k_deposit_gather<TP, TG, kahan><<<tiles, 256, 0, s>>>(N, g.h, bs.start, bs, ro, jx, jy, jz,
+                                                          bg_ro, ro1, jx1, jy1, jz1);
     return hipGetLastError();
 }
 

@@ -40,7 +40,23 @@ struct BinState {
     int *key;         // per particle: cell index or -1
     int *idx;         // per particle: particle ids sorted by cell
     unsigned *err;    // sticky error bits
+    void *rec;        // per particle, in cell order: deposit records (deposit.hip)
+    int64_t cap;      // particles the per-particle arrays hold
 };
+
+// What the deposit needs of one binned plasma particle (weights.py:35-67):
+// offsets xl, yl in [-1/2, 1/2) within its cell, dro = q / (1 - pz/gamma),
+// dro p / gamma, and the cell column iy; written at the particle's index by
+// the pass that bins it.
+template <typename T> struct alignas(16) DepRec {
+    T xl, yl, c0, c1, c2, c3;
+    int iy;
+};
+
+inline size_t deposit_record_bytes(bool fp32_bulk)
+{
+    return fp32_bulk ? sizeof(DepRec<float>) : sizeof(DepRec<double>);
+}
 
 // plasma3d/move.py:151-165 (free-streaming estimate).
 template <typename TP>

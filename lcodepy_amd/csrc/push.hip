@@ -46,21 +46,6 @@ __device__ __forceinline__ bool reflect(double x_init, double y_init, double b, 
     return hit;
 }
 
-// Cell of the deposit stencil centre for a position (weights.py:53-56); -1
-// (with an error bit) when the 5x5 stencil would leave the grid or the
-// position is not finite.
-__device__ __forceinline__ int deposit_cell(double x, double y, double h, int N, unsigned *err)
-{
-    int ix, iy;
-    double xl, yl;
-    bool finite;
-    if (!stencil_cell(x, y, h, N, ix, iy, xl, yl, finite)) {
-        atomicOr(err, finite ? ERR_PARTICLE_OFF_GRID : ERR_NONFINITE);
-        return -1;
-    }
-    return ix * N + iy;
-}
-
 template <typename TP, typename TF, bool FIRST, bool BIN>
 __global__ __launch_bounds__(256) void k_push(PushGeom g, Fields6<TF> f, SortArrays<TP> a,
                                               BinState bs)
@@ -196,7 +181,9 @@ __global__ __launch_bounds__(256) void k_push(PushGeom g, Fields6<TF> f, SortArr
     if (BIN) {
         int key = -1;
         if (!absorbed) {
-            key = deposit_cell(xi0 + xo, yi0 + yo, g.h, N, bs.err);
+            key = bin_particle<TP>(xi0 + xo, yi0 + yo, g.h, N, (double)px, (double)py,
+                                   (double)pz, m, (double)q, static_cast<DepRec<TP> *>(bs.rec),
+                                   k, bs.err);
             if (key >= 0) atomicAdd(&bs.count[key], 1);
         }
         bs.key[k] = key;
