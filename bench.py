@@ -3,14 +3,18 @@
 
 Workload (the 513 x 513, 4 ppc configuration): transverse window 25.6
 (step 0.05, N = 513), 4 plasma macro-particles per cell (964^2 = 929,296)
-with seeded thermal momenta ~ N(0, 1e-3), background ions, xi_step 0.02;
-a 1e6-particle seeded electron driver, n_b/n0 = 1, sigma_r = 0.5,
-sigma_xi = 1 (truncated at +-3 sigma), gamma ~ N(1000, 10).  One "step" is
-one xi step of the whole path: beam layer deposit, the plasma step (free
-stream, 3 pushes, 3 deposits, 2 field solves) and the beam push, with all
-inputs resident in HBM.  The FP32-bulk form is the headline value; the
-float64 form runs on the same inputs for the speed ratio and the on-axis Ez
-error.  Synthetic data (seeded), no files read.
+with seeded thermal momenta ~ N(0, 1e-3), background ions, xi_step 0.02 over
+a window of length 20 (1001 xi steps); a 1e6-particle seeded electron
+driver, n_b/n0 = 1, sigma_r = 0.5, sigma_xi = 1 (truncated at +-3 sigma),
+gamma ~ N(1000, 10).  One "step" is one xi step of the whole path: beam layer
+deposit, the plasma step (free stream, 3 pushes, 3 deposits, 2 field solves)
+and the beam push, with all inputs resident in HBM.  The FP32-bulk form is
+the headline value; the float64 form runs on the same inputs for the speed
+ratio and the on-axis Ez error over the same xi steps.  After the timed
+steps, the remaining steps of the window run with the library's per-launch
+event timing on (push, deposit gather, solve GEMM) for the roofline.
+Synthetic data (seeded), no files read except the traffic summary under
+profiles/ when present.
 """
 from __future__ import annotations
 
@@ -29,6 +33,9 @@ if ROOT not in sys.path:
 
 METRIC = ('xi-steps/s at 513² 4ppc, FP32-bulk vs FP64 @Ez err≤1e-4; HBM TB/s')
 HBM_PEAK = 8.0e12
+MFMA_PEAK = {'fp32-bulk': 157.3e12, 'fp64': 78.6e12}   # dense matrix-core FLOP/s
+WINDOW_STEPS = 1001                                     # xi steps of one time step
+TRAFFIC_FILE = os.path.join(ROOT, 'profiles', 'solve_gemm_513_4ppc_hbm_traffic.md')
 
 
 def workload_config(precision: str) -> dict:
@@ -111,39 +118,35 @@ def time_steps(run: Run, warmup: int, steps: int) -> float:
     return time.perf_counter() - t0
 
 
-def kernel_times(run: Run, launches: int) -> dict:
-    """Average duration of the push, the deposit and the field solve of the
-    live state, each launched alone between HIP events on the stream the
-    kernels use (torch's current stream)."""
-    import torch
-    solver = run.solver.plasma_solver
-    particles, fields, currents = run.solver._plasmastate
-    const_like = type('C', (), {})()
-    const_like.sorts = {'electrons': 0}
-    out = {}
+def live_kernel_times(run: Run, steps: int) -> dict:
+    """Average device seconds per launch of the push, the deposit gather and
+    the solve GEMM over `steps` further xi steps, from HIP events the library
+    records on the launch stream around those kernels."""
+    ctx = run.solver.plasma_solver.ctx
+    ctx.kernel_timing(True)
+    run.steps(steps)
+    times = ctx.kernel_times()
+    ctx.kernel_timing(False)
+    return times
 
-    def timed(fn):
-        fn()
-        torch.cuda.synchronize()
-        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        a.record()
-        for _ in range(launches):
-            fn()
-        b.record()
-        torch.cuda.synchronize()
-        return a.elapsed_time(b) / launches * 1e-3
 
-    full = {s: a.copy() for s, a in particles.items()}
-    out['push'] = timed(lambda: solver.move_particles(fields, particles, full, const_like))
-    deposit = solver.compute_rhoj
-    const = type('C', (), {})()
-    const.sorts = {'electrons': 0}
-    const.ro_initial = currents.ro[1]
-    out['deposit'] = timed(lambda: deposit(particles, const))
-    rho = torch.zeros(run.N, run.N, dtype=torch.float64, device=fields.Ex.device)
-    out['fields'] = timed(lambda: solver.compute_fields(fields, fields, None, rho, rho,
-                                                        currents, currents))
-    return out
+def gemm_flops(N: int) -> float:
+    """Flops of one solve-GEMM launch: the seven N x N products (one per
+    field) of a transform pass, 2 N^3 each."""
+    return 7 * 2.0 * N ** 3
+
+
+def hbm_traffic():
+    """HBM bytes per solve-GEMM launch from the rocprofv3 FETCH_SIZE and
+    WRITE_SIZE passes (scaled by a copy of known size), when summarised."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            for line in f:
+                if line.startswith('traffic_bytes_per_launch:'):
+                    return float(line.split(':', 1)[1])
+    except OSError:
+        pass
+    return None
 
 
 def push_bytes(n: int, N: int, bulk: int) -> int:
@@ -201,20 +204,41 @@ def cpu_baseline(sample_steps: int) -> dict:
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
-    ap.add_argument('--steps', type=int, default=100)
+    ap.add_argument('--steps', type=int, default=900)
     ap.add_argument('--warmup', type=int, default=10)
     ap.add_argument('--beam-particles', type=int, default=1_000_000)
-    ap.add_argument('--kernel-launches', type=int, default=20)
-    ap.add_argument('--cpu-sample-steps', type=int, default=3)
+    ap.add_argument('--timing-steps', type=int, default=None,
+                    help='xi steps with per-launch event timing after the timed steps '
+                         '(default: the rest of the window, at most 91)')
+    ap.add_argument('--cpu-sample-steps', type=int, default=20)
     ap.add_argument('--no-cpu-baseline', action='store_true')
     ap.add_argument('--fp64-only', action='store_true')
+    ap.add_argument('--dump-ez', default=None,
+                    help='write the on-axis Ez of every timed xi step of each form (npz)')
+    ap.add_argument('--copy-calibration-mib', type=int, default=256,
+                    help='size of the plain device copy run once before the timed steps, '
+                         'the reference dispatch for rocprofv3 FETCH_SIZE/WRITE_SIZE')
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get('WORLD_SIZE', '1')) != 1:
         print('bench.py measures one GPU: this build has no multi-GPU path '
               '(independent runs would shard by run; see DESIGN.md)', file=sys.stderr)
         return 2
+    timing_steps = args.timing_steps
+    if timing_steps is None:
+        timing_steps = max(1, min(91, WINDOW_STEPS - args.warmup - args.steps))
+    if args.warmup + args.steps + timing_steps > WINDOW_STEPS:
+        print(f'--warmup + --steps + --timing-steps must fit in the {WINDOW_STEPS} xi steps of '
+              'the window', file=sys.stderr)
+        return 2
     import torch
     torch.cuda.set_device(0)
+    if args.copy_calibration_mib > 0:
+        n = args.copy_calibration_mib * (1 << 20) // 4
+        src = torch.ones(n, dtype=torch.float32, device='cuda')
+        dst = torch.empty_like(src)
+        torch.mul(src, 1.0, out=dst)   # reads and writes copy_calibration_mib MiB
+        torch.cuda.synchronize()
+        del src, dst
 
     results = {}
     forms = ['fp64'] if args.fp64_only else ['fp32-bulk', 'fp64']
@@ -222,10 +246,12 @@ def main() -> int:
         print(f'[bench] {form}: building the workload', file=sys.stderr, flush=True)
         run = Run(form, args.beam_particles)
         elapsed = time_steps(run, args.warmup, args.steps)
+        ez = run.ez()
+        kt = live_kernel_times(run, timing_steps)
         run.solver.plasma_solver.ctx.raise_on_errors()
-        kt = kernel_times(run, args.kernel_launches)
-        results[form] = dict(elapsed=elapsed, ez=run.ez(), n=run.n_plasma, N=run.N, kt=kt)
-        print(f'[bench] {form}: {args.steps / elapsed:.1f} xi-steps/s, kernels {kt}',
+        results[form] = dict(elapsed=elapsed, ez=ez, n=run.n_plasma, N=run.N, kt=kt)
+        print(f'[bench] {form}: {args.steps / elapsed:.1f} xi-steps/s, per launch '
+              + ', '.join(f'{k} {v[0] * 1e6:.1f} us x{v[1]}' for k, v in kt.items()),
               file=sys.stderr, flush=True)
         del run
         torch.cuda.empty_cache()
@@ -233,9 +259,11 @@ def main() -> int:
     head = forms[0]
     r = results[head]
     value = args.steps / r['elapsed']
+    N, n = r['N'], r['n']
     bulk = 4 if head == 'fp32-bulk' else 8
-    pb = push_bytes(r['n'], r['N'], bulk)
-    t_push = r['kt']['push']
+    t_gemm, t_push, t_dep = (r['kt'][k][0] for k in ('solve_gemm', 'push', 'deposit_gather'))
+    fl = gemm_flops(N)
+    pb = push_bytes(n, N, bulk)
     line = {
         'metric': METRIC, 'value': value, 'unit': 'xi-steps/s', 'n_gpus': 1,
         'steps': args.steps, 'warmup': args.warmup,
@@ -245,23 +273,33 @@ def main() -> int:
         'data': 'synthetic (seeded plasma lattice with thermal momenta, seeded 1e6-particle '
                 'Gaussian driver)',
         'config': {'workload': '513x513 window 25.6, 4 ppc (929,296 plasma macro-particles), '
-                               'xi_step 0.02, 1e6-particle driver n_b=1 sigma_r=0.5 sigma_xi=1',
-                   'precision': head, 'N': r['N'], 'plasma_particles': r['n'],
+                               'xi_step 0.02 (1001 xi steps), 1e6-particle driver n_b=1 '
+                               'sigma_r=0.5 sigma_xi=1',
+                   'precision': head, 'N': N, 'plasma_particles': n,
                    'beam_particles': args.beam_particles},
-        'particle_pushes_per_s': 3 * r['n'] * value,
-        'kernel_avg_s': r['kt'],
-        'roofline': {'bound': 'hbm', 'kernel': 'k_push (one smart-push launch)',
-                     'achieved': pb / t_push / 1e9, 'peak': HBM_PEAK / 1e9, 'unit': 'GB/s',
-                     'frac': pb / t_push / HBM_PEAK, 'traffic': None},
+        'particle_pushes_per_s': 3 * n * value,
+        'kernel_avg_s': {k: v[0] for k, v in r['kt'].items()},
+        'kernel_launches_timed': {k: v[1] for k, v in r['kt'].items()},
+        'roofline': {'bound': 'mfma', 'kernel': 'k_gemm_mfma (solve transform pass, 7 fields)',
+                     'achieved': fl / t_gemm / 1e12, 'peak': MFMA_PEAK[head] / 1e12,
+                     'unit': 'TFLOP/s', 'frac': fl / t_gemm / MFMA_PEAK[head],
+                     'traffic': hbm_traffic()},
+        'roofline_hbm': {
+            'k_push': {'achieved': pb / t_push / 1e9, 'peak': HBM_PEAK / 1e9, 'unit': 'GB/s',
+                       'frac': pb / t_push / HBM_PEAK},
+        },
     }
     if 'fp64' in results and head != 'fp64':
         r64 = results['fp64']
         v64 = args.steps / r64['elapsed']
-        n = min(len(r64['ez']), len(r['ez']))
-        ez_err = float(np.max(np.abs(r['ez'][:n] - r64['ez'][:n])) / np.max(np.abs(r64['ez'][:n])))
+        k = min(len(r64['ez']), len(r['ez']))
+        ez_err = float(np.max(np.abs(r['ez'][:k] - r64['ez'][:k])) / np.max(np.abs(r64['ez'][:k])))
         line['fp64_xi_steps_per_s'] = v64
         line['fp32_bulk_over_fp64'] = value / v64
         line['on_axis_ez_rel_err_vs_fp64'] = ez_err
+        line['on_axis_ez_steps_compared'] = k
+    if args.dump_ez:
+        np.savez(args.dump_ez, **{form.replace('-', '_'): res['ez'] for form, res in results.items()})
     if not args.no_cpu_baseline:
         print('[bench] cpu baseline (oracle)', file=sys.stderr, flush=True)
         line['cpu_baseline'] = cpu_baseline(args.cpu_sample_steps)

@@ -22,10 +22,12 @@
  *     as plasma3d/initialization.py:391 lays them out.
  *   - Precision is fixed per context:
  *       LC3D_FP64       every array float64 (the reference's precision);
- *       LC3D_FP32_BULK  x_init, y_init, x_offt, y_offt and the fields of
+ *       LC3D_FP32_BULK  x_init, y_init, x_offt, y_offt, pz and the fields of
  *                       layers k / k+1 (fields_prev, fields_full) float64;
- *                       px, py, pz, q, m, currents and the half-step fields
- *                       float32.
+ *                       px, py, q, m, currents and the half-step fields
+ *                       float32.  pz stays float64 because gamma - pz, which
+ *                       sets 1/(1 - v_z) and the position update, cancels
+ *                       for forward-moving electrons.
  *     Beam arrays are float64 (ids and step counters int64, masks uint8) in
  *     both forms.
  *   - `stream` is a hipStream_t (NULL = default stream).  Work is
@@ -62,7 +64,7 @@ extern "C" {
 #define LC3D_ERR_PARTICLE_OFF_GRID 1u  /* plasma stencil left the grid: not deposited */
 #define LC3D_ERR_NONFINITE 2u          /* non-finite plasma position: not deposited */
 #define LC3D_ERR_BEAM_OFF_GRID 4u      /* beam stencil partly off the grid / substep cap hit */
-#define LC3D_ERR_CELL_OVERFLOW 8u      /* > 4096 particles in one cell (summed unsorted) */
+#define LC3D_ERR_CELL_OVERFLOW 8u      /* > 16384 particles in one cell (summed in arrival order) */
 
 typedef struct lc3d_ctx lc3d_ctx;
 
@@ -98,6 +100,18 @@ int lc3d_precision(const lc3d_ctx *ctx);
  * (synchronises `stream`). */
 int lc3d_error_flags(lc3d_ctx *ctx, uint32_t *flags, void *stream);
 
+/* Measurement, with no counterpart in the reference: device time per launch
+ * of the three heaviest kernels, from HIP events recorded on the launch
+ * stream around them.  enable != 0 starts it (and clears the totals); 0 stops
+ * it.  lc3d_kernel_times waits for the recorded events and gives, for
+ * [push, deposit gather, solve transform GEMM], the average seconds per
+ * launch and the launch counts since timing was enabled. */
+#define LC3D_KC_PUSH 0
+#define LC3D_KC_DEPOSIT_GATHER 1
+#define LC3D_KC_SOLVE_GEMM 2
+int lc3d_kernel_timing(lc3d_ctx *ctx, int32_t enable);
+int lc3d_kernel_times(lc3d_ctx *ctx, double seconds[3], int64_t launches[3]);
+
 /* ---- plasma particles ------------------------------------------------------ */
 typedef struct lc3d_fields6 {   /* the six fields a push reads */
     const void *Ex, *Ey, *Ez, *Bx, *By, *Bz;
@@ -108,15 +122,17 @@ typedef struct lc3d_sort {
     const double *x_init, *y_init;     /* constant lattice positions */
     const void *m;                     /* mass, constant */
     const double *x_offt_prev, *y_offt_prev;   /* layer k */
-    const void *px_prev, *py_prev, *pz_prev, *q_prev;
+    const void *px_prev, *py_prev, *q_prev;    /* bulk type */
+    const double *pz_prev;
     double *x_offt, *y_offt;                   /* layer k+1 (written) */
-    void *px, *py, *pz, *q;
+    void *px, *py, *q;                         /* bulk type */
+    double *pz;
     int32_t column;                    /* current column of this sort */
 } lc3d_sort;
 
 /* x_offt += px / (gamma - pz) * xi_step where m != 0 (in place). */
 int lc3d_move_wo_fields(lc3d_ctx *ctx, int64_t n, const void *m, double *x_offt, double *y_offt,
-                        const void *px, const void *py, const void *pz, void *stream);
+                        const void *px, const void *py, const double *pz, void *stream);
 
 /* Predictor-corrector push of one sort: reads the *_prev arrays and the
  * current estimate in x_offt / y_offt / q, writes x_offt, y_offt, px, py, pz

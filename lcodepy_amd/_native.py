@@ -23,8 +23,11 @@ ERR_BITS = {
     1: 'plasma particle stencil left the grid (not deposited)',
     2: 'non-finite plasma particle position (not deposited)',
     4: 'beam particle stencil partly off the grid or substep cap reached',
-    8: 'more than 4096 plasma particles in one cell (summed unsorted)',
+    8: 'more than 16384 plasma particles in one cell (summed in arrival order, so that '
+       'cell is reproducible only to rounding)',
 }
+# Bits that mean results were lost; the others are reported as warnings.
+FATAL_BITS = 1 | 2 | 4
 
 _vp = ctypes.c_void_p
 _dp = ctypes.c_void_p   # device pointers are passed as integers
@@ -61,14 +64,14 @@ class Sort(ctypes.Structure):
     _fields_ = [('n', ctypes.c_int64),
                 ('x_init', _vp), ('y_init', _vp), ('m', _vp),
                 ('x_offt_prev', _vp), ('y_offt_prev', _vp),
-                ('px_prev', _vp), ('py_prev', _vp), ('pz_prev', _vp), ('q_prev', _vp),
+                ('px_prev', _vp), ('py_prev', _vp), ('q_prev', _vp), ('pz_prev', _vp),
                 ('x_offt', _vp), ('y_offt', _vp),
-                ('px', _vp), ('py', _vp), ('pz', _vp), ('q', _vp),
+                ('px', _vp), ('py', _vp), ('q', _vp), ('pz', _vp),
                 ('column', ctypes.c_int32)]
 
 
 SYMBOLS = ('lc3d_create', 'lc3d_destroy', 'lc3d_last_error', 'lc3d_precision',
-           'lc3d_error_flags', 'lc3d_move_wo_fields', 'lc3d_move_smart', 'lc3d_deposit',
+           'lc3d_error_flags', 'lc3d_kernel_timing', 'lc3d_kernel_times', 'lc3d_move_wo_fields', 'lc3d_move_smart', 'lc3d_deposit',
            'lc3d_compute_fields', 'lc3d_plasma_step', 'lc3d_deposit_beam',
            'lc3d_beam_substep', 'lc3d_move_beam')
 
@@ -92,6 +95,8 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
     L.lc3d_last_error.restype = ctypes.c_char_p
     L.lc3d_precision.argtypes = [_vp]
     L.lc3d_error_flags.argtypes = [_vp, P(ctypes.c_uint32), _vp]
+    L.lc3d_kernel_timing.argtypes = [_vp, i32]
+    L.lc3d_kernel_times.argtypes = [_vp, P(dbl), P(i64)]
     L.lc3d_move_wo_fields.argtypes = [_vp, i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp]
     L.lc3d_move_smart.argtypes = [_vp, P(Sort), P(Fields6), i32, _vp]
     L.lc3d_deposit.argtypes = [_vp, P(Sort), _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]
@@ -196,8 +201,27 @@ class Context:
                    'lc3d_error_flags')
         return int(v.value)
 
+    KERNEL_CLASSES = ('push', 'deposit_gather', 'solve_gemm')
+
+    def kernel_timing(self, enable: bool) -> None:
+        """Start (and clear) or stop the per-launch event timing of the
+        push, the deposit gather and the solve's transform GEMMs."""
+        self.check(self.lib.lc3d_kernel_timing(self.handle, int(bool(enable))),
+                   'lc3d_kernel_timing')
+
+    def kernel_times(self) -> dict:
+        """{class: (seconds per launch, launches)} since timing was enabled."""
+        sec = (ctypes.c_double * 3)()
+        cnt = (ctypes.c_int64 * 3)()
+        self.check(self.lib.lc3d_kernel_times(self.handle, sec, cnt), 'lc3d_kernel_times')
+        return {k: (sec[i], cnt[i]) for i, k in enumerate(self.KERNEL_CLASSES)}
+
     def raise_on_errors(self, stream=None) -> None:
         flags = self.error_flags(stream)
-        if flags:
-            msgs = [m for b, m in ERR_BITS.items() if flags & b]
+        if flags & ~FATAL_BITS:
+            import warnings
+            warnings.warn('lcodepy_amd kernels reported: ' + '; '.join(
+                m for b, m in ERR_BITS.items() if flags & b & ~FATAL_BITS))
+        if flags & FATAL_BITS:
+            msgs = [m for b, m in ERR_BITS.items() if flags & b & FATAL_BITS]
             raise FloatingPointError('lcodepy_amd kernels reported: ' + '; '.join(msgs))

@@ -23,6 +23,12 @@ struct lc3d_ctx {
     int *count = nullptr, *start = nullptr, *blocks = nullptr, *key = nullptr, *idx = nullptr;
     unsigned *err = nullptr;
     void *rec = nullptr;
+    int *big = nullptr, *nbig = nullptr;
+    int bigcap = 0;
+    // idx is a permutation of the sort with this x_init and size after its
+    // last deposit; pushes of that sort walk it (cell order).
+    const void *order_x_init = nullptr;
+    int64_t order_n = -1;
     // field solve
     int solver = LC3D_SOLVER_MATRIX;
     void *work = nullptr, *work2 = nullptr;
@@ -30,6 +36,7 @@ struct lc3d_ctx {
     void *tw = nullptr;                                      // FFT twiddles
     void *tdst = nullptr, *tdct = nullptr, *tdct_t = nullptr;   // transform matrices
     double *lam = nullptr;
+    KernelTimer timer;
     std::string last_error;
 };
 
@@ -100,7 +107,7 @@ PushGeom push_geom(const lc3d_ctx *c, int64_t n)
 BinState bin_state(const lc3d_ctx *c)
 {
     return BinState{c->count, c->start, c->blocks, c->key, c->idx, c->err, c->rec,
-                    std::max<int64_t>(1, c->cap)};
+                    std::max<int64_t>(1, c->cap), c->big, c->nbig, c->bigcap};
 }
 
 template <typename TP> SortArrays<TP> sort_arrays(const lc3d_sort *s)
@@ -113,13 +120,13 @@ template <typename TP> SortArrays<TP> sort_arrays(const lc3d_sort *s)
     a.yo_prev = s->y_offt_prev;
     a.px_prev = static_cast<const TP *>(s->px_prev);
     a.py_prev = static_cast<const TP *>(s->py_prev);
-    a.pz_prev = static_cast<const TP *>(s->pz_prev);
+    a.pz_prev = static_cast<const double *>(s->pz_prev);
     a.q_prev = static_cast<const TP *>(s->q_prev);
     a.xo = s->x_offt;
     a.yo = s->y_offt;
     a.px = static_cast<TP *>(s->px);
     a.py = static_cast<TP *>(s->py);
-    a.pz = static_cast<TP *>(s->pz);
+    a.pz = static_cast<double *>(s->pz);
     a.q = static_cast<TP *>(s->q);
     return a;
 }
@@ -152,6 +159,11 @@ MatrixGeom matrix_geom(const lc3d_ctx *c)
                       c->lam, c->tdst, c->tdct, c->tdct_t, c->work, c->work2, c->rowsum};
 }
 
+KernelTimer *timer_of(const lc3d_ctx *c)
+{
+    return c->timer.on ? const_cast<KernelTimer *>(&c->timer) : nullptr;
+}
+
 // Dispatches the solve to the context's implementation.
 template <typename TG, typename TPRED, typename TFULL, typename THALF>
 hipError_t run_solve(const lc3d_ctx *c, const SolveIn<TG, TPRED> &in,
@@ -161,8 +173,8 @@ hipError_t run_solve(const lc3d_ctx *c, const SolveIn<TG, TPRED> &in,
     if (c->solver == LC3D_SOLVER_FFT)
         return launch_compute_fields<TG, TPRED, TFULL, THALF>(solve_geom(c), in, out, write_full, st);
 #endif
-    return launch_compute_fields_matrix<TG, TPRED, TFULL, THALF>(matrix_geom(c), in, out,
-                                                                 write_full, st);
+    return launch_compute_fields_matrix<TG, TPRED, TFULL, THALF>(
+        matrix_geom(c), in, out, write_full, st, timer_of(c));
 }
 
 template <typename TG, typename TPRED>
@@ -237,7 +249,7 @@ int deposit_sort(lc3d_ctx *c, const lc3d_sort *s, bool binned, const lc3d_curren
                                          at(cur->jy, col), at(cur->jz, col),
                                          bg ? static_cast<const float *>(bg_ro) : nullptr,
                                          at(cur->ro, 1), at(cur->jx, 1), at(cur->jy, 1),
-                                         at(cur->jz, 1), st);
+                                         at(cur->jz, 1), st, timer_of(c));
     } else {
         auto at = [&](void *p, int k) { return static_cast<double *>(p) + k * nn; };
         e = launch_deposit<double, double>(push_geom(c, s->n), sort_arrays<double>(s), true, binned,
@@ -245,10 +257,18 @@ int deposit_sort(lc3d_ctx *c, const lc3d_sort *s, bool binned, const lc3d_curren
                                            at(cur->jy, col), at(cur->jz, col),
                                            bg ? static_cast<const double *>(bg_ro) : nullptr,
                                            at(cur->ro, 1), at(cur->jx, 1), at(cur->jy, 1),
-                                           at(cur->jz, 1), st);
+                                           at(cur->jz, 1), st, timer_of(c));
     }
     if (e != hipSuccess) return hip_fail(c, e, "deposit");
+    c->order_x_init = s->x_init;
+    c->order_n = s->n;
     return LC3D_OK;
+}
+
+// The cell order of the last deposit, when it belongs to this sort.
+const int *push_order(const lc3d_ctx *c, const lc3d_sort *s)
+{
+    return (s->n > 0 && c->order_n == s->n && c->order_x_init == s->x_init) ? c->idx : nullptr;
 }
 
 int push_sort(lc3d_ctx *c, const lc3d_sort *s, const void *const *f, bool f32, bool first,
@@ -259,14 +279,17 @@ int push_sort(lc3d_ctx *c, const lc3d_sort *s, const void *const *f, bool f32, b
     if (fp32(c)) {
         if (f32)
             e = launch_push<float, float>(push_geom(c, s->n), f6<float>(&fl), sort_arrays<float>(s),
-                                          first, bin, bin_state(c), st);
+                                          first, bin, bin_state(c), push_order(c, s), st,
+                                          timer_of(c));
         else
             e = launch_push<float, double>(push_geom(c, s->n), f6<double>(&fl),
-                                           sort_arrays<float>(s), first, bin, bin_state(c), st);
+                                           sort_arrays<float>(s), first, bin, bin_state(c),
+                                           push_order(c, s), st, timer_of(c));
     } else {
         if (f32) return fail(c, LC3D_EINVAL, "float32 fields need an FP32-bulk context");
         e = launch_push<double, double>(push_geom(c, s->n), f6<double>(&fl),
-                                        sort_arrays<double>(s), first, bin, bin_state(c), st);
+                                        sort_arrays<double>(s), first, bin, bin_state(c),
+                                        push_order(c, s), st, timer_of(c));
     }
     if (e != hipSuccess) return hip_fail(c, e, "push");
     return LC3D_OK;
@@ -345,13 +368,15 @@ int lc3d_create(const lc3d_params *p, lc3d_ctx **out)
     const bool f32 = p->precision == LC3D_FP32_BULK;
     const size_t tg = f32 ? sizeof(float) : sizeof(double);
     const size_t cap = (size_t)std::max<int64_t>(1, c->cap);
-    bool ok = hipMalloc(&c->count, nn * sizeof(int)) == hipSuccess &&
+    bool ok = hipMalloc(&c->count, (nn + 1) * sizeof(int)) == hipSuccess &&
               hipMalloc(&c->start, (nn + 1) * sizeof(int)) == hipSuccess &&
               hipMalloc(&c->blocks, nblocks * sizeof(int)) == hipSuccess &&
               hipMalloc(&c->key, cap * sizeof(int)) == hipSuccess &&
               hipMalloc(&c->idx, cap * sizeof(int)) == hipSuccess &&
               hipMalloc(&c->err, sizeof(unsigned)) == hipSuccess &&
               hipMalloc(&c->rec, cap * deposit_record_bytes(f32)) == hipSuccess &&
+              hipMalloc(&c->big, (cap / 64 + 2) * sizeof(int)) == hipSuccess &&
+              hipMalloc(&c->nbig, sizeof(int)) == hipSuccess &&
               hipMalloc(&c->work, 7 * nn * tg) == hipSuccess &&
               hipMalloc(&c->rowsum, N * sizeof(double)) == hipSuccess &&
               hipMalloc(&c->lam, N * sizeof(double)) == hipSuccess;
@@ -413,8 +438,10 @@ int lc3d_create(const lc3d_params *p, lc3d_ctx **out)
         if (e == hipSuccess) e = upload(c->tdct_t, Ct);
     }
     if (e == hipSuccess) e = hipMemcpy(c->lam, lam.data(), N * sizeof(double), hipMemcpyHostToDevice);
-    if (e == hipSuccess) e = hipMemset(c->count, 0, nn * sizeof(int));
+    if (e == hipSuccess) e = hipMemset(c->count, 0, (nn + 1) * sizeof(int));
     if (e == hipSuccess) e = hipMemset(c->err, 0, sizeof(unsigned));
+    if (e == hipSuccess) e = hipMemset(c->nbig, 0, sizeof(int));
+    c->bigcap = (int)(cap / 64 + 2);
     if (e == hipSuccess) e = hipDeviceSynchronize();
     if (e != hipSuccess) {
         lc3d_destroy(c);
@@ -427,6 +454,10 @@ int lc3d_create(const lc3d_params *p, lc3d_ctx **out)
 void lc3d_destroy(lc3d_ctx *c)
 {
     if (!c) return;
+    if (c->timer.on) {
+        (void)c->timer.harvest();
+        for (hipEvent_t &e : c->timer.ev) (void)hipEventDestroy(e);
+    }
     (void)hipFree(c->count);
     (void)hipFree(c->start);
     (void)hipFree(c->blocks);
@@ -434,6 +465,8 @@ void lc3d_destroy(lc3d_ctx *c)
     (void)hipFree(c->idx);
     (void)hipFree(c->err);
     (void)hipFree(c->rec);
+    (void)hipFree(c->big);
+    (void)hipFree(c->nbig);
     (void)hipFree(c->work);
     (void)hipFree(c->work2);
     (void)hipFree(c->rowsum);
@@ -449,6 +482,38 @@ const char *lc3d_last_error(const lc3d_ctx *c) { return c ? c->last_error.c_str(
 
 int lc3d_precision(const lc3d_ctx *c) { return c ? c->prm.precision : -1; }
 
+int lc3d_kernel_timing(lc3d_ctx *c, int32_t enable)
+{
+    if (!c) return LC3D_EINVAL;
+    KernelTimer &t = c->timer;
+    if (t.on) {
+        HIP_TRY(c, t.harvest());
+        for (hipEvent_t &e : t.ev) (void)hipEventDestroy(e);
+        t.on = false;
+    }
+    if (enable) {
+        for (hipEvent_t &e : t.ev) HIP_TRY(c, hipEventCreate(&e));
+        for (int k = 0; k < KC_COUNT; ++k) {
+            t.seconds[k] = 0;
+            t.launches[k] = 0;
+        }
+        t.used = 0;
+        t.on = true;
+    }
+    return LC3D_OK;
+}
+
+int lc3d_kernel_times(lc3d_ctx *c, double *seconds, int64_t *launches)
+{
+    if (!c || !seconds || !launches) return LC3D_EINVAL;
+    if (c->timer.on) HIP_TRY(c, c->timer.harvest());
+    for (int k = 0; k < KC_COUNT; ++k) {
+        launches[k] = c->timer.launches[k];
+        seconds[k] = launches[k] ? c->timer.seconds[k] / (double)launches[k] : 0.0;
+    }
+    return LC3D_OK;
+}
+
 int lc3d_error_flags(lc3d_ctx *c, uint32_t *flags, void *stream)
 {
     if (!c || !flags) return LC3D_EINVAL;
@@ -463,7 +528,7 @@ int lc3d_error_flags(lc3d_ctx *c, uint32_t *flags, void *stream)
 }
 
 int lc3d_move_wo_fields(lc3d_ctx *c, int64_t n, const void *m, double *x_offt, double *y_offt,
-                        const void *px, const void *py, const void *pz, void *stream)
+                        const void *px, const void *py, const double *pz, void *stream)
 {
     if (!c || n < 0 || (n > 0 && (!m || !x_offt || !y_offt || !px || !py || !pz)))
         return fail(c, LC3D_EINVAL, "move_wo_fields: bad arguments");
@@ -473,12 +538,12 @@ int lc3d_move_wo_fields(lc3d_ctx *c, int64_t n, const void *m, double *x_offt, d
         e = launch_move_wo_fields<float>(n, c->prm.xi_step_size, static_cast<const float *>(m),
                                          x_offt, y_offt, static_cast<const float *>(px),
                                          static_cast<const float *>(py),
-                                         static_cast<const float *>(pz), st);
+                                         pz, st);
     else
         e = launch_move_wo_fields<double>(n, c->prm.xi_step_size, static_cast<const double *>(m),
                                           x_offt, y_offt, static_cast<const double *>(px),
                                           static_cast<const double *>(py),
-                                          static_cast<const double *>(pz), st);
+                                          pz, st);
     if (e != hipSuccess) return hip_fail(c, e, "move_wo_fields");
     return LC3D_OK;
 }
@@ -506,7 +571,7 @@ int lc3d_deposit(lc3d_ctx *c, const lc3d_sort *s, void *ro, void *jx, void *jy, 
                                          static_cast<float *>(jx), static_cast<float *>(jy),
                                          static_cast<float *>(jz), static_cast<const float *>(bg_ro),
                                          static_cast<float *>(ro1), static_cast<float *>(jx1),
-                                         static_cast<float *>(jy1), static_cast<float *>(jz1), st);
+                                         static_cast<float *>(jy1), static_cast<float *>(jz1), st, timer_of(c));
     else
         e = launch_deposit<double, double>(push_geom(c, s->n), sort_arrays<double>(s), true, false,
                                            bin_state(c), static_cast<double *>(ro),
@@ -515,8 +580,10 @@ int lc3d_deposit(lc3d_ctx *c, const lc3d_sort *s, void *ro, void *jx, void *jy, 
                                            static_cast<const double *>(bg_ro),
                                            static_cast<double *>(ro1), static_cast<double *>(jx1),
                                            static_cast<double *>(jy1), static_cast<double *>(jz1),
-                                           st);
+                                           st, timer_of(c));
     if (e != hipSuccess) return hip_fail(c, e, "deposit");
+    c->order_x_init = s->x_init;
+    c->order_n = s->n;
     return LC3D_OK;
 }
 

@@ -13,8 +13,8 @@ enum : unsigned {
     ERR_NONFINITE = 2u,           // a non-finite plasma particle position
     ERR_BEAM_OFF_GRID = 4u,       // a beam particle's stencil left the grid (or
                                   // its substep count exceeded the cap)
-    ERR_CELL_OVERFLOW = 8u,       // > 4096 plasma particles in one cell: that
-                                  // cell's list is summed unsorted
+    ERR_CELL_OVERFLOW = 8u,       // > 16384 plasma particles in one cell: that
+                                  // cell's list is summed in arrival order
 };
 
 template <typename T> struct cplx;

@@ -23,7 +23,7 @@
 #include "common.h"
 #include "kernels.h"
 
-#include <type_traits>
+#include <climits>
 
 namespace lc3d {
 
@@ -32,7 +32,8 @@ constexpr int DEP_HALO = DEP_TILE + 4;
 constexpr int SCAN_THREADS = 256;
 constexpr int SCAN_PER = 8;
 constexpr int SCAN_TILE = SCAN_THREADS * SCAN_PER;
-constexpr int MAX_SORTED_CELL = 4096;
+constexpr int SMALL_CELL = 64;          // lists sorted by their cell's thread
+constexpr int MAX_SORTED_CELL = 16384;  // lists sorted by a workgroup in LDS
 
 // ---- binning ------------------------------------------------------------------
 template <typename TP>
@@ -42,7 +43,7 @@ __global__ __launch_bounds__(256) void k_bin(int64_t n, double h, int N,
                                              const double *__restrict__ xo,
                                              const double *__restrict__ yo,
                                              const TP *__restrict__ px, const TP *__restrict__ py,
-                                             const TP *__restrict__ pz, const TP *__restrict__ q,
+                                             const double *__restrict__ pz, const TP *__restrict__ q,
                                              const TP *__restrict__ m, BinState bs)
 {
     const int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
@@ -122,25 +123,38 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_scan_final(int ncell, int nblo
 }
 
 // ---- scatter ids into the cell lists, counting the counters back to zero --------
-__global__ __launch_bounds__(256) void k_scatter(int64_t n, BinState bs)
+// Places particle k in its cell's list; particles that are not binned (q = 0
+// or off the grid) go after all the lists, in arrival order, so that idx is a
+// permutation of the sort.  count[ncell] counts them and is cleared by
+// k_order_cells.
+__global__ __launch_bounds__(256) void k_scatter(int64_t n, int ncell, BinState bs)
 {
     const int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (k >= n) return;
+    if (k == 0) *bs.nbig = 0;   // k_order_cells refills the big-cell list
     const int key = bs.key[k];
-    if (key < 0) return;
+    if (key < 0) {
+        bs.idx[bs.start[ncell] + atomicAdd(&bs.count[ncell], 1)] = (int)k;
+        return;
+    }
     const int slot = bs.start[key] + atomicSub(&bs.count[key], 1) - 1;
     bs.idx[slot] = (int)k;
 }
 
 // ---- sort each cell's list by particle id ---------------------------------------
+// Lists of up to SMALL_CELL particles are sorted by their cell's thread
+// (insertion sort); longer ones are queued for k_order_big.
 __global__ __launch_bounds__(256) void k_order_cells(int ncell, BinState bs)
 {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c == 0) bs.count[ncell] = 0;   // the scatter's count of unbinned particles
     if (c >= ncell) return;
     const int s = bs.start[c], e = bs.start[c + 1];
     if (e - s < 2) return;
-    if (e - s > MAX_SORTED_CELL) {   // degenerate pile-up: sum in arrival order
-        atomicOr(bs.err, ERR_CELL_OVERFLOW);
+    if (e - s > SMALL_CELL) {
+        const int slot = e - s <= MAX_SORTED_CELL ? atomicAdd(bs.nbig, 1) : bs.bigcap;
+        if (slot < bs.bigcap) bs.big[slot] = c;
+        else atomicOr(bs.err, ERR_CELL_OVERFLOW);   // summed in arrival order
         return;
     }
     int *a = bs.idx;
@@ -155,11 +169,46 @@ __global__ __launch_bounds__(256) void k_order_cells(int ncell, BinState bs)
     }
 }
 
+// Bitonic sort, in LDS, of the queued long lists: one workgroup per list at
+// a time, the grid striding over the queue.
+constexpr int BIG_SORT_THREADS = 256;
+__global__ __launch_bounds__(BIG_SORT_THREADS) void k_order_big(BinState bs)
+{
+    __shared__ int s[MAX_SORTED_CELL];
+    const int nbig = min(*bs.nbig, bs.bigcap);
+    const int t = threadIdx.x;
+    for (int b = blockIdx.x; b < nbig; b += gridDim.x) {
+        const int c = bs.big[b];
+        const int st = bs.start[c], n = bs.start[c + 1] - st;   // SMALL_CELL < n <= MAX_SORTED_CELL
+        int P = 1;
+        while (P < n) P <<= 1;
+        for (int i = t; i < P; i += BIG_SORT_THREADS) s[i] = i < n ? bs.idx[st + i] : INT_MAX;
+        __syncthreads();
+        for (int k = 2; k <= P; k <<= 1) {
+            for (int j = k >> 1; j > 0; j >>= 1) {
+                for (int i = t; i < P; i += BIG_SORT_THREADS) {
+                    const int l = i ^ j;
+                    if (l > i) {
+                        const int x = s[i], y = s[l];
+                        if ((x > y) == ((i & k) == 0)) {
+                            s[i] = y;
+                            s[l] = x;
+                        }
+                    }
+                }
+                __syncthreads();
+            }
+        }
+        for (int i = t; i < n; i += BIG_SORT_THREADS) bs.idx[st + i] = s[i];
+        __syncthreads();
+    }
+}
+
 // ---- gather deposit -------------------------------------------------------------------
-// Particle slots per halo cell staged at once.
-template <typename T> struct SlotCap;
-template <> struct SlotCap<double> { static constexpr int value = 2; };
-template <> struct SlotCap<float> { static constexpr int value = 4; };
+// Particles staged per LDS chunk.
+template <typename T> struct GatherCap;
+template <> struct GatherCap<double> { static constexpr int value = 256; };
+template <> struct GatherCap<float> { static constexpr int value = 512; };
 
 template <typename T, bool KAHAN> struct Acc;
 template <typename T> struct Acc<T, false> {
@@ -179,14 +228,13 @@ template <typename T> struct Acc<T, true> {
 };
 
 // One block per 16 x 16 tile of output cells, one thread per output cell.
-// The records (written by the pass that binned each particle) of the 20 x 20
-// halo cells' particles are staged into LDS slot by slot
-// ([slot][halo row][halo column]), so the 64 lanes of a wave read 16
-// consecutive cells of 4 rows and all run the same 25 x slots loop.  Each
-// output cell sums its 5 x 5 source cells row by row (x ascending), then by
-// column, then in particle-id order within a cell; passes over slots K at a
-// time interleave this order when a halo cell holds more than K particles.
-// No floating-point atomics: the result is reproducible bit for bit.
+// The block stages the records (written by the pass that binned each
+// particle) of its 20 x 20 halo cells' particles into LDS, CAP at a time, in
+// cell order; each output cell then sums, source row by source row (x
+// ascending), the particles of its 5 source cells in that row (cell order,
+// then particle id).  Work follows the local particle count, so clustered
+// plasma costs what it deposits.  No floating-point atomics: the result is
+// reproducible bit for bit.
 template <typename TP, typename TG, bool KAHAN>
 __global__ __launch_bounds__(256) void k_deposit_gather(
     int N, double h, const int *__restrict__ start, BinState bs, TG *__restrict__ ro,
@@ -194,98 +242,100 @@ __global__ __launch_bounds__(256) void k_deposit_gather(
     TG *__restrict__ ro1, TG *__restrict__ jx1, TG *__restrict__ jy1, TG *__restrict__ jz1)
 {
     using T = TP;   // arithmetic type of the tile
-    constexpr int K = SlotCap<T>::value;
-    constexpr int H = DEP_HALO;
-    __shared__ int s_start[H][H];
-    __shared__ int s_cnt[H][H];
-    __shared__ T s_xl[K][H][H];
-    __shared__ T s_yl[K][H][H];
-    __shared__ T s_c[4][K][H][H];
-    __shared__ int s_max;
+    constexpr int CAP = GatherCap<T>::value;
+    __shared__ int hs[DEP_HALO][DEP_HALO + 1];   // global list position of each halo cell
+    __shared__ int rowoff[DEP_HALO + 1];         // halo-list offset of each halo row
+    __shared__ T s_wx[5][CAP];
+    __shared__ T s_wy[5][CAP];
+    __shared__ T s_c[4][CAP];
+    __shared__ int s_cy[CAP];
 
     const int tile = xcd_swizzle(blockIdx.x, gridDim.x);
     const int tpr = (N + DEP_TILE - 1) / DEP_TILE;
     const int X0 = (tile / tpr) * DEP_TILE, Y0 = (tile % tpr) * DEP_TILE;
     const DepRec<T> *__restrict__ rec = static_cast<const DepRec<T> *>(bs.rec);
+    const int hx0 = max(0, X0 - 2), hx1 = min(N, X0 + DEP_TILE + 2);
+    const int hy0 = max(0, Y0 - 2), hy1 = min(N, Y0 + DEP_TILE + 2);
+    const int nrow = hx1 - hx0, ncol = hy1 - hy0;
     const int t = threadIdx.x;
-    if (t == 0) s_max = 0;
-    __syncthreads();
-    int my_max = 0;
-    for (int i = t; i < H * H; i += 256) {
-        const int r = i / H, c = i % H;
-        const int gx = X0 - 2 + r, gy = Y0 - 2 + c;
-        int st = 0, cnt = 0;
-        if (gx >= 0 && gx < N && gy >= 0 && gy < N) {
-            const int cell = gx * N + gy;
-            st = start[cell];
-            cnt = start[cell + 1] - st;
-        }
-        s_start[r][c] = st;
-        s_cnt[r][c] = cnt;
-        my_max = max(my_max, cnt);
-    }
-    atomicMax(&s_max, my_max);
-    __syncthreads();
-    const int maxcnt = s_max;
 
+    for (int i = t; i < nrow * (ncol + 1); i += 256) {
+        const int r = i / (ncol + 1), c = i % (ncol + 1);
+        hs[r][c] = start[(hx0 + r) * N + hy0 + c];
+    }
+    __syncthreads();
+    if (t == 0) {
+        int acc = 0;
+        for (int r = 0; r < nrow; ++r) {
+            rowoff[r] = acc;
+            acc += hs[r][ncol] - hs[r][0];
+        }
+        rowoff[nrow] = acc;
+    }
+    __syncthreads();
+    const int total = rowoff[nrow];
+
+    // This thread's output cell and, per source row, its particle range.
     const int tx = t / DEP_TILE, ty = t % DEP_TILE;
     const int ox = X0 + tx, oy = Y0 + ty;
+    const bool valid = ox < N && oy < N;
+    int lo[5], hi[5];
+#pragma unroll
+    for (int kr = 0; kr < 5; ++kr) {
+        const int r = ox - 2 + kr - hx0;
+        lo[kr] = hi[kr] = 0;
+        if (valid && r >= 0 && r < nrow) {
+            const int c0 = max(oy - 2, hy0) - hy0, c1 = min(oy + 3, hy1) - hy0;
+            lo[kr] = rowoff[r] + hs[r][c0] - hs[r][0];
+            hi[kr] = rowoff[r] + hs[r][c1] - hs[r][0];
+        }
+    }
+
     Acc<T, KAHAN> a_ro, a_jx, a_jy, a_jz;
     const T inv_h = T(1.0 / h);
-    for (int k0 = 0; k0 < maxcnt; k0 += K) {
-        for (int e = t; e < K * H * H; e += 256) {
-            const int k = e / (H * H), cell = e % (H * H);
-            const int r = cell / H, c = cell % H;
-            T xl = 0, yl = 0, c0 = 0, c1 = 0, c2 = 0, c3 = 0;
-            if (k0 + k < s_cnt[r][c]) {
-                const DepRec<T> d = rec[bs.idx[s_start[r][c] + k0 + k]];
-                xl = d.xl;
-                yl = d.yl;
-                c0 = d.c0;
-                c1 = d.c1;
-                c2 = d.c2;
-                c3 = d.c3;
+    for (int k0 = 0; k0 < total; k0 += CAP) {
+        const int k1 = min(total, k0 + CAP);
+        // Stage: quartic weights (pre-divided by h) and current factors per
+        // particle, from the records in cell order.
+        for (int j = k0 + t; j < k1; j += 256) {
+            int lo_r = 0, hi_r = nrow;   // rowoff[lo_r] <= j < rowoff[hi_r]
+            while (hi_r - lo_r > 1) {
+                const int mid = (lo_r + hi_r) >> 1;
+                if (rowoff[mid] <= j) lo_r = mid;
+                else hi_r = mid;
             }
-            s_xl[k][r][c] = xl;
-            s_yl[k][r][c] = yl;
-            s_c[0][k][r][c] = c0;
-            s_c[1][k][r][c] = c1;
-            s_c[2][k][r][c] = c2;
-            s_c[3][k][r][c] = c3;
+            const DepRec<T> d = rec[bs.idx[hs[lo_r][0] + (j - rowoff[lo_r])]];
+            T w[5];
+            weights4<T>(d.xl, w);
+#pragma unroll
+            for (int i = 0; i < 5; ++i) s_wx[i][j - k0] = w[i] * inv_h;
+            weights4<T>(d.yl, w);
+#pragma unroll
+            for (int i = 0; i < 5; ++i) s_wy[i][j - k0] = w[i] * inv_h;
+            s_c[0][j - k0] = d.c0;
+            s_c[1][j - k0] = d.c1;
+            s_c[2][j - k0] = d.c2;
+            s_c[3][j - k0] = d.c3;
+            s_cy[j - k0] = d.iy;
         }
         __syncthreads();
-        // Source cell (tx + sx, ty + sy) in halo coordinates lies at offset
-        // (2 - sx, 2 - sy) from this output cell: taps 4 - sx and 4 - sy.
-        const int kn = min(K, maxcnt - k0);
-        auto row = [&](auto sxc) {
-            constexpr int sx = decltype(sxc)::value;
-            auto col = [&](auto syc) {
-                constexpr int sy = decltype(syc)::value;
-                for (int k = 0; k < kn; ++k) {
-                    const T wx = w4_tap<4 - sx>(s_xl[k][tx + sx][ty + sy]) * inv_h;
-                    const T wy = w4_tap<4 - sy>(s_yl[k][tx + sx][ty + sy]) * inv_h;
-                    const T w = wx * wy;
-                    a_ro.add(s_c[0][k][tx + sx][ty + sy] * w);
-                    a_jx.add(s_c[1][k][tx + sx][ty + sy] * w);
-                    a_jy.add(s_c[2][k][tx + sx][ty + sy] * w);
-                    a_jz.add(s_c[3][k][tx + sx][ty + sy] * w);
-                }
-            };
-            col(std::integral_constant<int, 0>{});
-            col(std::integral_constant<int, 1>{});
-            col(std::integral_constant<int, 2>{});
-            col(std::integral_constant<int, 3>{});
-            col(std::integral_constant<int, 4>{});
-        };
-        row(std::integral_constant<int, 0>{});
-        row(std::integral_constant<int, 1>{});
-        row(std::integral_constant<int, 2>{});
-        row(std::integral_constant<int, 3>{});
-        row(std::integral_constant<int, 4>{});
+        // Accumulate source rows in ascending order: kx = ox - sx = 2 - kr.
+#pragma unroll
+        for (int kr = 0; kr < 5; ++kr) {
+            const int b = max(lo[kr], k0), e = min(hi[kr], k1);
+            for (int j = b; j < e; ++j) {
+                const int jj = j - k0;
+                const T w = s_wx[4 - kr][jj] * s_wy[oy - s_cy[jj] + 2][jj];
+                a_ro.add(s_c[0][jj] * w);
+                a_jx.add(s_c[1][jj] * w);
+                a_jy.add(s_c[2][jj] * w);
+                a_jz.add(s_c[3][jj] * w);
+            }
+        }
         __syncthreads();
     }
 
-    if (ox < N && oy < N) {
+    if (valid) {
         const int o = ox * N + oy;
         ro[o] = (TG)a_ro.get();
         jx[o] = (TG)a_jx.get();
@@ -303,7 +353,7 @@ __global__ __launch_bounds__(256) void k_deposit_gather(
 template <typename TP, typename TG>
 hipError_t launch_deposit(const PushGeom &g, const SortArrays<TP> &a, bool use_full, bool binned,
                           const BinState &bs, TG *ro, TG *jx, TG *jy, TG *jz, const TG *bg_ro,
-                          TG *ro1, TG *jx1, TG *jy1, TG *jz1, hipStream_t s)
+                          TG *ro1, TG *jx1, TG *jy1, TG *jz1, hipStream_t s, KernelTimer *timer)
 {
     const int N = g.N;
     const int ncell = N * N;
@@ -311,7 +361,7 @@ hipError_t launch_deposit(const PushGeom &g, const SortArrays<TP> &a, bool use_f
     const double *yo = use_full ? a.yo : a.yo_prev;
     const TP *px = use_full ? a.px : a.px_prev;
     const TP *py = use_full ? a.py : a.py_prev;
-    const TP *pz = use_full ? a.pz : a.pz_prev;
+    const double *pz = use_full ? a.pz : a.pz_prev;
     const TP *q = use_full ? a.q : a.q_prev;
     const int pblocks = (int)((g.n + 255) / 256);
     if (g.n > 0) {
@@ -321,8 +371,9 @@ hipError_t launch_deposit(const PushGeom &g, const SortArrays<TP> &a, bool use_f
         const int nb = (ncell + 1 + SCAN_TILE - 1) / SCAN_TILE;
         k_scan_blocks<<<nb, SCAN_THREADS, 0, s>>>(ncell, bs.count, bs.blocks);
         k_scan_final<<<nb, SCAN_THREADS, 0, s>>>(ncell, nb, bs.count, bs.blocks, bs.start);
-        k_scatter<<<pblocks, 256, 0, s>>>(g.n, bs);
+        k_scatter<<<pblocks, 256, 0, s>>>(g.n, ncell, bs);
         k_order_cells<<<(ncell + 255) / 256, 256, 0, s>>>(ncell, bs);
+        k_order_big<<<256, BIG_SORT_THREADS, 0, s>>>(bs);
     } else {
         // Empty sort: every cell list is empty.
         const int nb = (ncell + 1 + SCAN_TILE - 1) / SCAN_TILE;
@@ -332,19 +383,23 @@ hipError_t launch_deposit(const PushGeom &g, const SortArrays<TP> &a, bool use_f
     const int tpr = (N + DEP_TILE - 1) / DEP_TILE;
     const int tiles = tpr * tpr;
     constexpr bool kahan = sizeof(TP) == 4;
+    hipError_t e;
+    if (timer && (e = timer->begin(KC_DEPOSIT_GATHER, 1, s)) != hipSuccess) return e;
     k_deposit_gather<TP, TG, kahan><<<tiles, 256, 0, s>>>(N, g.h, bs.start, bs, ro, jx, jy, jz,
                                                           bg_ro, ro1, jx1, jy1, jz1);
-    return hipGetLastError();
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    if (timer) return timer->end(s);
+    return hipSuccess;
 }
 
 template hipError_t launch_deposit<double, double>(const PushGeom &, const SortArrays<double> &,
                                                    bool, bool, const BinState &, double *,
                                                    double *, double *, double *, const double *,
                                                    double *, double *, double *, double *,
-                                                   hipStream_t);
+                                                   hipStream_t, KernelTimer *);
 template hipError_t launch_deposit<float, float>(const PushGeom &, const SortArrays<float> &,
                                                  bool, bool, const BinState &, float *, float *,
                                                  float *, float *, const float *, float *,
-                                                 float *, float *, float *, hipStream_t);
+                                                 float *, float *, float *, hipStream_t, KernelTimer *);
 
 }  // namespace lc3d

@@ -1,8 +1,10 @@
 // Host-side launch interfaces of the lcodepy_amd kernels (gfx950).
 //
-// TP is the bulk particle type (px, py, pz, q, m): double in the FP64 form,
-// float in the FP32-bulk form.  Positions (x_init, y_init, x_offt, y_offt)
-// are always double.  TF is the type of the field arrays a push reads, TG the
+// TP is the bulk particle type (px, py, q, m): double in the FP64 form, float
+// in the FP32-bulk form.  Positions (x_init, y_init, x_offt, y_offt) and pz
+// are always double: gamma - pz, which sets 1/(1 - v_z) and the position
+// update, cancels catastrophically for forward-moving electrons unless pz is
+// kept in float64.  TF is the type of the field arrays a push reads, TG the
 // type of the current / spectral grids.
 #pragma once
 #include <hip/hip_runtime.h>
@@ -27,21 +29,26 @@ template <typename TP> struct SortArrays {
     const double *x_init, *y_init;
     const TP *m;
     const double *xo_prev, *yo_prev;
-    const TP *px_prev, *py_prev, *pz_prev, *q_prev;
+    const TP *px_prev, *py_prev, *q_prev;
+    const double *pz_prev;
     double *xo, *yo;
-    TP *px, *py, *pz, *q;
+    TP *px, *py, *q;
+    double *pz;
 };
 
 // Binning state shared by push and deposit (owned by the context).
 struct BinState {
-    int *count;       // N*N, zero between deposits
+    int *count;       // N*N + 1, zero between deposits
     int *start;       // N*N + 1
     int *blocks;      // scan block sums
     int *key;         // per particle: cell index or -1
-    int *idx;         // per particle: particle ids sorted by cell
+    int *idx;         // per particle: particle ids sorted by cell, then the unbinned ones
     unsigned *err;    // sticky error bits
-    void *rec;        // per particle, in cell order: deposit records (deposit.hip)
+    void *rec;        // per particle: deposit records (deposit.hip)
     int64_t cap;      // particles the per-particle arrays hold
+    int *big;         // cells whose list is sorted by a whole workgroup
+    int *nbig;        // how many cells `big` holds
+    int bigcap;       // capacity of `big`
 };
 
 // What the deposit needs of one binned plasma particle (weights.py:35-67):
@@ -58,17 +65,63 @@ inline size_t deposit_record_bytes(bool fp32_bulk)
     return fp32_bulk ? sizeof(DepRec<float>) : sizeof(DepRec<double>);
 }
 
+// Device time per launch of the heaviest kernels, measured with HIP events
+// recorded on the launch stream around them (off unless enabled on the
+// context).  An interval may span several back-to-back launches of a class.
+enum KernelClass { KC_PUSH = 0, KC_DEPOSIT_GATHER = 1, KC_SOLVE_GEMM = 2, KC_COUNT = 3 };
+
+struct KernelTimer {
+    static constexpr int CAP = 64;   // intervals recorded before they are read back
+    bool on = false;
+    hipEvent_t ev[2 * CAP] = {};
+    int cls[CAP] = {}, nl[CAP] = {};
+    int used = 0;
+    double seconds[KC_COUNT] = {};
+    int64_t launches[KC_COUNT] = {};
+
+    hipError_t begin(int c, int nlaunch, hipStream_t s) {
+        cls[used] = c;
+        nl[used] = nlaunch;
+        return hipEventRecord(ev[2 * used], s);
+    }
+    hipError_t end(hipStream_t s) {
+        hipError_t e = hipEventRecord(ev[2 * used + 1], s);
+        if (e != hipSuccess) return e;
+        if (++used == CAP) return harvest();
+        return hipSuccess;
+    }
+    // Waits for the recorded intervals and adds them up.
+    hipError_t harvest() {
+        for (int i = 0; i < used; ++i) {
+            hipError_t e = hipEventSynchronize(ev[2 * i + 1]);
+            if (e != hipSuccess) return e;
+            float ms = 0;
+            e = hipEventElapsedTime(&ms, ev[2 * i], ev[2 * i + 1]);
+            if (e != hipSuccess) return e;
+            seconds[cls[i]] += 1e-3 * ms;
+            launches[cls[i]] += nl[i];
+        }
+        used = 0;
+        return hipSuccess;
+    }
+};
+
 // plasma3d/move.py:151-165 (free-streaming estimate).
 template <typename TP>
 hipError_t launch_move_wo_fields(int64_t n, double xi_step, const TP *m, double *xo, double *yo,
-                                 const TP *px, const TP *py, const TP *pz, hipStream_t s);
+                                 const TP *px, const TP *py, const double *pz, hipStream_t s);
 
 // plasma3d/move.py:52-148.  first=true folds in the free-streaming estimate
 // from the previous layer (move.py:331-349); bin=true also records each
-// particle's deposit cell for the deposit that follows.
+// particle's deposit cell for the deposit that follows.  `order`, when not
+// null, is a permutation of the n particles (the last deposit's cell order):
+// thread j pushes particle order[j], so neighbouring threads read
+// neighbouring field cells.  Each particle is pushed independently, so the
+// result does not depend on the order.
 template <typename TP, typename TF>
 hipError_t launch_push(const PushGeom &g, const Fields6<TF> &f, const SortArrays<TP> &a,
-                       bool first, bool bin, const BinState &bs, hipStream_t s);
+                       bool first, bool bin, const BinState &bs, const int *order, hipStream_t s,
+                       KernelTimer *timer = nullptr);
 
 // plasma3d/weights.py:35-67, 249-285 for one sort.  When `binned` is false
 // the cells are computed here; otherwise the preceding push recorded them.
@@ -77,7 +130,8 @@ hipError_t launch_push(const PushGeom &g, const Fields6<TF> &f, const SortArrays
 template <typename TP, typename TG>
 hipError_t launch_deposit(const PushGeom &g, const SortArrays<TP> &a, bool use_full, bool binned,
                           const BinState &bs, TG *ro, TG *jx, TG *jy, TG *jz, const TG *bg_ro,
-                          TG *ro1, TG *jx1, TG *jy1, TG *jz1, hipStream_t s);
+                          TG *ro1, TG *jx1, TG *jy1, TG *jz1, hipStream_t s,
+                          KernelTimer *timer = nullptr);
 
 // ---- field solve (plasma3d/fields.py:112-322) -------------------------------
 struct FftPlan {
@@ -130,7 +184,7 @@ struct MatrixGeom {
 template <typename TG, typename TPRED, typename TFULL, typename THALF>
 hipError_t launch_compute_fields_matrix(const MatrixGeom &g, const SolveIn<TG, TPRED> &in,
                                         const SolveOut<TFULL, THALF> &out, bool write_full,
-                                        hipStream_t s);
+                                        hipStream_t s, KernelTimer *timer = nullptr);
 
 // ---- beam (beam3d) -----------------------------------------------------------
 struct BeamGeom {

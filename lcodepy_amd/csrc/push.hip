@@ -20,7 +20,7 @@ __global__ __launch_bounds__(256) void k_move_wo_fields(int64_t n, double xi_ste
                                                         double *__restrict__ yo,
                                                         const TP *__restrict__ px,
                                                         const TP *__restrict__ py,
-                                                        const TP *__restrict__ pz)
+                                                        const double *__restrict__ pz)
 {
     const int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (k >= n) return;
@@ -48,16 +48,18 @@ __device__ __forceinline__ bool reflect(double x_init, double y_init, double b, 
 
 template <typename TP, typename TF, bool FIRST, bool BIN>
 __global__ __launch_bounds__(256) void k_push(PushGeom g, Fields6<TF> f, SortArrays<TP> a,
-                                              BinState bs)
+                                              BinState bs, const int *__restrict__ order)
 {
-    const int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    if (k >= g.n) return;
+    const int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (j >= g.n) return;
+    const int64_t k = order ? (int64_t)order[j] : j;   // the particle this thread pushes
     const int N = g.N;
     const double b = g.boundary, dxi = g.xi_step;
 
     const double xi0 = a.x_init[k], yi0 = a.y_init[k];
     const double xop = a.xo_prev[k], yop = a.yo_prev[k];
-    const TP opx = a.px_prev[k], opy = a.py_prev[k], opz = a.pz_prev[k];
+    const TP opx = a.px_prev[k], opy = a.py_prev[k];
+    const double opz = a.pz_prev[k];
     const double m = (double)a.m[k];
 
     TP q;
@@ -95,8 +97,12 @@ __global__ __launch_bounds__(256) void k_push(PushGeom g, Fields6<TF> f, SortArr
     // Midpoint between the previous position and the current estimate.
     double x = xi0 + (xop + xof) * 0.5, y = yi0 + (yop + yof) * 0.5;
     double xo_tmp = 0, yo_tmp = 0;
-    TP px = opx, py = opy, pz = opz;
-    if (reflect(xi0, yi0, b, x, y, xo_tmp, yo_tmp)) px = py = pz = TP(0);
+    TP px = opx, py = opy;
+    double pz = opz;
+    if (reflect(xi0, yi0, b, x, y, xo_tmp, yo_tmp)) {
+        px = py = TP(0);
+        pz = 0;
+    }
 
     // 5x5 gather of the six fields at the midpoint (move.py:90-107).
     TP Ex = 0, Ey = 0, Ez = 0, Bx = 0, By = 0, Bz = 0;
@@ -143,7 +149,7 @@ __global__ __launch_bounds__(256) void k_push(PushGeom g, Fields6<TF> f, SortArr
         dpz = factor * (Ez + vx * By - vy * Bx);
         px = opx + dpx * TP(0.5);
         py = opy + dpy * TP(0.5);
-        pz = opz + dpz * TP(0.5);
+        pz = opz + (double)(dpz * TP(0.5));
     }
 
     // Position from the half-step momentum, then the full-step momentum.
@@ -157,17 +163,21 @@ __global__ __launch_bounds__(256) void k_push(PushGeom g, Fields6<TF> f, SortArr
     }
     px = opx + dpx;
     py = opy + dpy;
-    pz = opz + dpz;
+    pz = opz + (double)dpz;
 
     x = xi0 + xo;
     y = yi0 + yo;
-    if (reflect(xi0, yi0, b, x, y, xo, yo)) px = py = pz = TP(0);
+    if (reflect(xi0, yi0, b, x, y, xo, yo)) {
+        px = py = TP(0);
+        pz = 0;
+    }
     x = xi0 + xo;
     y = yi0 + yo;
     bool absorbed = false;
     if (x >= b || x <= -b || y >= b || y <= -b) {
         xo = yo = 0;
-        px = py = pz = TP(0);
+        px = py = TP(0);
+        pz = 0;
         absorbed = true;
     }
 
@@ -192,7 +202,7 @@ __global__ __launch_bounds__(256) void k_push(PushGeom g, Fields6<TF> f, SortArr
 
 template <typename TP>
 hipError_t launch_move_wo_fields(int64_t n, double xi_step, const TP *m, double *xo, double *yo,
-                                 const TP *px, const TP *py, const TP *pz, hipStream_t s)
+                                 const TP *px, const TP *py, const double *pz, hipStream_t s)
 {
     if (n <= 0) return hipSuccess;
     const int blocks = (int)((n + 255) / 256);
@@ -202,15 +212,20 @@ hipError_t launch_move_wo_fields(int64_t n, double xi_step, const TP *m, double 
 
 template <typename TP, typename TF>
 hipError_t launch_push(const PushGeom &g, const Fields6<TF> &f, const SortArrays<TP> &a,
-                       bool first, bool bin, const BinState &bs, hipStream_t s)
+                       bool first, bool bin, const BinState &bs, const int *order, hipStream_t s,
+                       KernelTimer *timer)
 {
     if (g.n <= 0) return hipSuccess;
     const int blocks = (int)((g.n + 255) / 256);
-    if (first && bin) k_push<TP, TF, true, true><<<blocks, 256, 0, s>>>(g, f, a, bs);
-    else if (first) k_push<TP, TF, true, false><<<blocks, 256, 0, s>>>(g, f, a, bs);
-    else if (bin) k_push<TP, TF, false, true><<<blocks, 256, 0, s>>>(g, f, a, bs);
-    else k_push<TP, TF, false, false><<<blocks, 256, 0, s>>>(g, f, a, bs);
-    return hipGetLastError();
+    hipError_t e;
+    if (timer && (e = timer->begin(KC_PUSH, 1, s)) != hipSuccess) return e;
+    if (first && bin) k_push<TP, TF, true, true><<<blocks, 256, 0, s>>>(g, f, a, bs, order);
+    else if (first) k_push<TP, TF, true, false><<<blocks, 256, 0, s>>>(g, f, a, bs, order);
+    else if (bin) k_push<TP, TF, false, true><<<blocks, 256, 0, s>>>(g, f, a, bs, order);
+    else k_push<TP, TF, false, false><<<blocks, 256, 0, s>>>(g, f, a, bs, order);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    if (timer) return timer->end(s);
+    return hipSuccess;
 }
 
 template hipError_t launch_move_wo_fields<double>(int64_t, double, const double *, double *,
@@ -218,15 +233,15 @@ template hipError_t launch_move_wo_fields<double>(int64_t, double, const double 
                                                   const double *, hipStream_t);
 template hipError_t launch_move_wo_fields<float>(int64_t, double, const float *, double *,
                                                  double *, const float *, const float *,
-                                                 const float *, hipStream_t);
+                                                 const double *, hipStream_t);
 template hipError_t launch_push<double, double>(const PushGeom &, const Fields6<double> &,
                                                 const SortArrays<double> &, bool, bool,
-                                                const BinState &, hipStream_t);
+                                                const BinState &, const int *, hipStream_t, KernelTimer *);
 template hipError_t launch_push<float, double>(const PushGeom &, const Fields6<double> &,
                                                const SortArrays<float> &, bool, bool,
-                                               const BinState &, hipStream_t);
+                                               const BinState &, const int *, hipStream_t, KernelTimer *);
 template hipError_t launch_push<float, float>(const PushGeom &, const Fields6<float> &,
                                               const SortArrays<float> &, bool, bool,
-                                              const BinState &, hipStream_t);
+                                              const BinState &, const int *, hipStream_t, KernelTimer *);
 
 }  // namespace lc3d

@@ -178,7 +178,7 @@ __global__ __launch_bounds__(256) void k_assemble(const TG *__restrict__ work, i
 template <typename TG, typename TPRED, typename TFULL, typename THALF>
 hipError_t launch_compute_fields_matrix(const MatrixGeom &g, const SolveIn<TG, TPRED> &in,
                                         const SolveOut<TFULL, THALF> &out, bool write_full,
-                                        hipStream_t s)
+                                        hipStream_t s, KernelTimer *timer)
 {
     const int N = g.N;
     const size_t nn = (size_t)N * N;
@@ -201,10 +201,13 @@ hipError_t launch_compute_fields_matrix(const MatrixGeom &g, const SolveIn<TG, T
     }
     const dim3 tiles((N + GBN - 1) / GBN, (N + GBM - 1) / GBM, 7);
     k_rhs<TG, TPRED><<<dim3((unsigned)((nn + 255) / 256), 7), 256, 0, s>>>(rhs, w1);
+    hipError_t e;
+    if (timer && (e = timer->begin(KC_SOLVE_GEMM, 4, s)) != hipSuccess) return e;
     k_gemm_mfma<TG, false><<<tiles, 256, 0, s>>>(fy, N, sg);
     k_gemm_mfma<TG, true><<<tiles, 256, 0, s>>>(sx, N, sg);
     k_gemm_mfma<TG, false><<<tiles, 256, 0, s>>>(ix, N, sg);
     k_gemm_mfma<TG, false><<<tiles, 256, 0, s>>>(iy, N, sg);
+    if (timer && (e = timer->end(s)) != hipSuccess) return e;
     k_assemble<TG, TFULL, THALF><<<dim3((N + 3) / 4, 7), 256, 0, s>>>(w1, N, out, write_full,
                                                                        g.rowsum);
     const int fb = (int)std::min<int64_t>(1024, ((int64_t)N * N + 255) / 256);
@@ -215,12 +218,12 @@ hipError_t launch_compute_fields_matrix(const MatrixGeom &g, const SolveIn<TG, T
 
 template hipError_t launch_compute_fields_matrix<double, double, double, double>(
     const MatrixGeom &, const SolveIn<double, double> &, const SolveOut<double, double> &, bool,
-    hipStream_t);
+    hipStream_t, KernelTimer *);
 template hipError_t launch_compute_fields_matrix<float, double, double, float>(
     const MatrixGeom &, const SolveIn<float, double> &, const SolveOut<double, float> &, bool,
-    hipStream_t);
+    hipStream_t, KernelTimer *);
 template hipError_t launch_compute_fields_matrix<float, float, double, float>(
     const MatrixGeom &, const SolveIn<float, float> &, const SolveOut<double, float> &, bool,
-    hipStream_t);
+    hipStream_t, KernelTimer *);
 
 }  // namespace lc3d

@@ -10,7 +10,8 @@ PARTICLE_KEYS = ('x_init', 'y_init', 'x_offt', 'y_offt', 'px', 'py', 'pz', 'q', 
 def check_sort(a, dt) -> None:
     for k in PARTICLE_KEYS:
         t = getattr(a, k)
-        want = dt.position if k in ('x_init', 'y_init', 'x_offt', 'y_offt') else dt.bulk
+        want = (dt.position if k in ('x_init', 'y_init', 'x_offt', 'y_offt')
+                else dt.pz if k == 'pz' else dt.bulk)
         if t.dtype != want or not t.is_cuda or not t.is_contiguous():
             raise TypeError(f'particle array {k}: need a contiguous {want} device tensor, '
                             f'got {t.dtype} on {t.device}')

@@ -82,7 +82,8 @@ def particles_to_device(config: Config, host: dict) -> Arrays:
     dev = runtime.device()
     out = {}
     for k, v in host.items():
-        dtype = dt.position if k in ('x_init', 'y_init', 'x_offt', 'y_offt') else dt.bulk
+        dtype = (dt.position if k in ('x_init', 'y_init', 'x_offt', 'y_offt')
+                 else dt.pz if k == 'pz' else dt.bulk)
         if k.startswith('d') and 'chaotic' in k:
             dtype = dt.position
         out[k] = torch.as_tensor(np.ascontiguousarray(v)).to(device=dev, dtype=dtype)

@@ -34,7 +34,8 @@ class Dtypes:
         self.precision = precision
         bulk = precision == 'fp32-bulk'
         self.position = torch.float64          # x_init, y_init, x_offt, y_offt
-        self.bulk = torch.float32 if bulk else torch.float64   # px py pz q m
+        self.bulk = torch.float32 if bulk else torch.float64   # px py q m
+        self.pz = torch.float64                # gamma - pz must not cancel
         self.full = torch.float64              # fields of layers k, k+1
         self.half = torch.float32 if bulk else torch.float64   # half-step fields
         self.current = torch.float32 if bulk else torch.float64

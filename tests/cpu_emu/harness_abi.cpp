@@ -39,7 +39,7 @@ struct Run {
 };
 
 template <typename TP> struct Sort {
-    double *xi, *yi, *xo, *yo, *xo1, *yo1; TP *m, *px, *py, *pz, *q, *px1, *py1, *pz1, *q1;
+    double *xi, *yi, *xo, *yo, *xo1, *yo1; TP *m, *px, *py, *q, *px1, *py1, *q1; double *pz, *pz1;
     lc3d_sort s{};
     Sort(Run &r, double spread, double pmax) {
         int64_t n = r.n;
@@ -47,8 +47,8 @@ template <typename TP> struct Sort {
         for (int64_t a = 0; a < r.side; ++a) for (int64_t b = 0; b < r.side; ++b) { xi[a * r.side + b] = r.grid[a]; yi[a * r.side + b] = r.grid[b]; }
         xo = arr<double>(n); yo = arr<double>(n); xo1 = arr<double>(n); yo1 = arr<double>(n);
         for (int64_t k = 0; k < n; ++k) { xo[k] = uni(-spread, spread) * r.h; yo[k] = uni(-spread, spread) * r.h; }
-        m = arr<TP>(n); q = arr<TP>(n); px = arr<TP>(n, pmax); py = arr<TP>(n, pmax); pz = arr<TP>(n, pmax);
-        px1 = arr<TP>(n); py1 = arr<TP>(n); pz1 = arr<TP>(n); q1 = arr<TP>(n);
+        m = arr<TP>(n); q = arr<TP>(n); px = arr<TP>(n, pmax); py = arr<TP>(n, pmax); pz = arr<double>(n, pmax);
+        px1 = arr<TP>(n); py1 = arr<TP>(n); pz1 = arr<double>(n); q1 = arr<TP>(n);
         for (int64_t k = 0; k < n; ++k) { m[k] = (TP)(r.h * r.h / 4); q[k] = (TP)(uni(0, 1) < 0.01 ? 0.0 : -r.h * r.h / 4); }
         s.n = n; s.x_init = xi; s.y_init = yi; s.m = m; s.x_offt_prev = xo; s.y_offt_prev = yo;
         s.px_prev = px; s.py_prev = py; s.pz_prev = pz; s.q_prev = q;
@@ -80,7 +80,7 @@ template <typename TP, typename TG, typename TH> void run_all(int N, int prec, i
     Sort<TP> s(r, 0.3, 0.05);
     TG *ro = arr<TG>(nn), *jx = arr<TG>(nn), *jy = arr<TG>(nn), *jz = arr<TG>(nn);
     std::memcpy(s.xo1, s.xo, r.n * 8); std::memcpy(s.yo1, s.yo, r.n * 8);
-    std::memcpy(s.px1, s.px, r.n * sizeof(TP)); std::memcpy(s.py1, s.py, r.n * sizeof(TP)); std::memcpy(s.pz1, s.pz, r.n * sizeof(TP)); std::memcpy(s.q1, s.q, r.n * sizeof(TP));
+    std::memcpy(s.px1, s.px, r.n * sizeof(TP)); std::memcpy(s.py1, s.py, r.n * sizeof(TP)); std::memcpy(s.pz1, s.pz, r.n * sizeof(double)); std::memcpy(s.q1, s.q, r.n * sizeof(TP));
     CHECK(lc3d_deposit(r.ctx, &s.s, ro, jx, jy, jz, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr));
     printf("deposit ok\n"); fflush(stdout);
     CHECK(lc3d_move_wo_fields(r.ctx, r.n, s.m, s.xo1, s.yo1, s.px1, s.py1, s.pz1, nullptr));
@@ -89,9 +89,15 @@ template <typename TP, typename TG, typename TH> void run_all(int N, int prec, i
     printf("push ok\n"); fflush(stdout);
     lc3d_currents cf2 = currents<TG>(N, 0.0);
     TG *bg = arr<TG>(nn, 0.01);
+    CHECK(lc3d_kernel_timing(r.ctx, 1));
     for (int step = 0; step < 2; ++step)
         CHECK(lc3d_plasma_step(r.ctx, 1, &s.s, &prev, &cp, rbf, rbp, bg, &full, &half, &cf2, nullptr));
-    printf("plasma_step 1 sort ok\n"); fflush(stdout);
+    double sec[3]; int64_t nl[3];
+    CHECK(lc3d_kernel_times(r.ctx, sec, nl));
+    CHECK(lc3d_kernel_timing(r.ctx, 0));
+    const int64_t want_gemm = solver == LC3D_SOLVER_MATRIX ? 16 : 0;
+    if (nl[0] != 6 || nl[1] != 6 || nl[2] != want_gemm) { printf("FAIL kernel launch counts %lld %lld %lld\n", (long long)nl[0], (long long)nl[1], (long long)nl[2]); exit(1); }
+    printf("plasma_step 1 sort ok (timed launches %lld %lld %lld)\n", (long long)nl[0], (long long)nl[1], (long long)nl[2]); fflush(stdout);
     Sort<TP> ions(r, 0.3, 0.01); ions.s.column = 1;
     lc3d_sort two[2] = {s.s, ions.s};
     CHECK(lc3d_plasma_step(r.ctx, 2, two, &prev, &cp, rbf, rbp, nullptr, &full, &half, &cf2, nullptr));

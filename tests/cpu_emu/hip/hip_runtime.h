@@ -8,6 +8,7 @@
 // so every out-of-range global or LDS access is an ASan report.  Device
 // allocations are host mallocs of the exact size.
 #pragma once
+#include <chrono>
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
@@ -82,3 +83,14 @@ inline hipError_t hipMemsetAsync(void *d, int v, size_t n, hipStream_t) { std::m
 inline hipError_t hipDeviceSynchronize() { return 0; }
 inline hipError_t hipStreamSynchronize(hipStream_t) { return 0; }
 inline hipError_t hipGetDevice(int *d) { *d = 0; return 0; }
+// Events: launches run to completion when issued, so an event is the host
+// time at which it was recorded.
+typedef double *hipEvent_t;
+inline hipError_t hipEventCreate(hipEvent_t *e) { *e = new double(0); return 0; }
+inline hipError_t hipEventDestroy(hipEvent_t e) { delete e; return 0; }
+inline hipError_t hipEventRecord(hipEvent_t e, hipStream_t) {
+    *e = std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+    return 0;
+}
+inline hipError_t hipEventSynchronize(hipEvent_t) { return 0; }
+inline hipError_t hipEventElapsedTime(float *ms, hipEvent_t a, hipEvent_t b) { *ms = (float)(1e3 * (*b - *a)); return 0; }

@@ -114,7 +114,7 @@ def _random_plasma(rng, p, spread=0.3, pmax=0.05):
 
 def _sort_arrays(a, dtype=torch.float64):
     from lcodepy_amd.plasma3d.data import Arrays
-    pos = ('x_init', 'y_init', 'x_offt', 'y_offt')
+    pos = ('x_init', 'y_init', 'x_offt', 'y_offt', 'pz')
     return Arrays.wrap(**{k: dev(v, torch.float64 if k in pos else dtype) for k, v in a.items()})
 
 
@@ -139,6 +139,45 @@ def test_deposit_matches_oracle_and_is_deterministic(N):
         assert torch.equal(u, v)                       # bit-reproducible
         assert relerr(host(u[0]), w) < 1e-12   # cell-order vs particle-order sums
         assert not torch.any(u[1])
+
+
+@pytest.mark.parametrize('N,frac', [(65, 0.3), (129, 0.45)])
+def test_deposit_of_collapsed_plasma(N, frac):
+    """A fraction of the particles collapsed into the few cells around the
+    axis (thousands per cell at N=65: the workgroup sort of long cell lists;
+    over 16384 in one cell at N=129: arrival-order summation, reported as a
+    warning bit).  Against the oracle within 1e-12; bitwise reproducible
+    unless a cell overflows."""
+    import warnings
+    from lcodepy_amd.plasma3d.data import Arrays
+    from lcodepy_amd.plasma3d.weights import get_deposit_plasma
+    from lcodepy_amd import runtime
+    print(f'collapsed deposit N={N} frac={frac}', flush=True)
+    rng = np.random.default_rng(30 + N)
+    p = _params(N)
+    a = _random_plasma(rng, p, spread=0.3)
+    pick = rng.uniform(size=a['x_init'].shape) < frac
+    a['x_offt'][pick] = -a['x_init'][pick] + rng.uniform(-0.45, 0.45, pick.sum()) * p.h
+    a['y_offt'][pick] = -a['y_init'][pick] + rng.uniform(-0.45, 0.45, pick.sum()) * p.h
+    g = [np.zeros((N, N)) for _ in range(4)]
+    O.deposit_sort(p, a, *g)
+    cfg = _cfg(N)
+    deposit = get_deposit_plasma(cfg)
+    const = Arrays.wrap()
+    const.sorts = {'electrons': 0}
+    parts = {'electrons': _sort_arrays(a)}
+    out1 = deposit(parts, const)
+    out2 = deposit(parts, const)
+    torch.cuda.synchronize()
+    ctx = runtime.context(cfg)
+    flags = ctx.error_flags()
+    overflow = bool(flags & 8)
+    assert overflow == (N == 129), flags
+    assert not flags & ~8, flags
+    for u, v, w in zip(out1, out2, g):
+        if not overflow:
+            assert torch.equal(u, v)
+        assert relerr(host(u[0]), w) < 1e-12
 
 
 @pytest.mark.parametrize('N', [65, 129])
