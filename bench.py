@@ -298,6 +298,10 @@ def main() -> int:
         line['fp32_bulk_over_fp64'] = value / v64
         line['on_axis_ez_rel_err_vs_fp64'] = ez_err
         line['on_axis_ez_steps_compared'] = k
+        scale = np.max(np.abs(r64['ez'][:k]))
+        line['on_axis_ez_rel_err_first_steps'] = {
+            str(m): float(np.max(np.abs(r['ez'][:m] - r64['ez'][:m])) / scale)
+            for m in (100, 200, 300, 400, 500, 700) if m <= k}
     if args.dump_ez:
         np.savez(args.dump_ez, **{form.replace('-', '_'): res['ez'] for form, res in results.items()})
     if not args.no_cpu_baseline:

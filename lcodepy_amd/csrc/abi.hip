@@ -23,7 +23,8 @@ struct lc3d_ctx {
     int *count = nullptr, *start = nullptr, *blocks = nullptr, *key = nullptr, *idx = nullptr;
     unsigned *err = nullptr;
     void *rec = nullptr;
-    int *big = nullptr, *nbig = nullptr;
+    int *big = nullptr, *nbig = nullptr, *dslot = nullptr;
+    void *dense = nullptr;
     int bigcap = 0;
     // idx is a permutation of the sort with this x_init and size after its
     // last deposit; pushes of that sort walk it (cell order).
@@ -107,7 +108,8 @@ PushGeom push_geom(const lc3d_ctx *c, int64_t n)
 BinState bin_state(const lc3d_ctx *c)
 {
     return BinState{c->count, c->start, c->blocks, c->key, c->idx, c->err, c->rec,
-                    std::max<int64_t>(1, c->cap), c->big, c->nbig, c->bigcap};
+                    std::max<int64_t>(1, c->cap), c->big, c->nbig, c->bigcap, c->dslot,
+                    c->dense};
 }
 
 template <typename TP> SortArrays<TP> sort_arrays(const lc3d_sort *s)
@@ -377,6 +379,8 @@ int lc3d_create(const lc3d_params *p, lc3d_ctx **out)
               hipMalloc(&c->rec, cap * deposit_record_bytes(f32)) == hipSuccess &&
               hipMalloc(&c->big, (cap / 64 + 2) * sizeof(int)) == hipSuccess &&
               hipMalloc(&c->nbig, sizeof(int)) == hipSuccess &&
+              hipMalloc(&c->dslot, nn * sizeof(int)) == hipSuccess &&
+              hipMalloc(&c->dense, (cap / 64 + 2) * 100 * tg) == hipSuccess &&
               hipMalloc(&c->work, 7 * nn * tg) == hipSuccess &&
               hipMalloc(&c->rowsum, N * sizeof(double)) == hipSuccess &&
               hipMalloc(&c->lam, N * sizeof(double)) == hipSuccess;
@@ -467,6 +471,8 @@ void lc3d_destroy(lc3d_ctx *c)
     (void)hipFree(c->rec);
     (void)hipFree(c->big);
     (void)hipFree(c->nbig);
+    (void)hipFree(c->dslot);
+    (void)hipFree(c->dense);
     (void)hipFree(c->work);
     (void)hipFree(c->work2);
     (void)hipFree(c->rowsum);

@@ -143,20 +143,24 @@ __global__ __launch_bounds__(256) void k_scatter(int64_t n, int ncell, BinState 
 
 // ---- sort each cell's list by particle id ---------------------------------------
 // Lists of up to SMALL_CELL particles are sorted by their cell's thread
-// (insertion sort); longer ones are queued for k_order_big.
+// (insertion sort).  Longer ones make the cell dense: it is queued (its slot
+// recorded in dslot) for k_order_big and k_dense_cells.
 __global__ __launch_bounds__(256) void k_order_cells(int ncell, BinState bs)
 {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c == 0) bs.count[ncell] = 0;   // the scatter's count of unbinned particles
     if (c >= ncell) return;
     const int s = bs.start[c], e = bs.start[c + 1];
-    if (e - s < 2) return;
+    int slot = -1;
     if (e - s > SMALL_CELL) {
-        const int slot = e - s <= MAX_SORTED_CELL ? atomicAdd(bs.nbig, 1) : bs.bigcap;
+        // At most n / (SMALL_CELL + 1) < bigcap cells can be dense.
+        slot = atomicAdd(bs.nbig, 1);
         if (slot < bs.bigcap) bs.big[slot] = c;
-        else atomicOr(bs.err, ERR_CELL_OVERFLOW);   // summed in arrival order
-        return;
+        else slot = -1;
+        if (e - s > MAX_SORTED_CELL) atomicOr(bs.err, ERR_CELL_OVERFLOW);
     }
+    bs.dslot[c] = slot;
+    if (slot >= 0 || e - s < 2) return;
     int *a = bs.idx;
     for (int i = s + 1; i < e; ++i) {
         const int v = a[i];
@@ -179,7 +183,8 @@ __global__ __launch_bounds__(BIG_SORT_THREADS) void k_order_big(BinState bs)
     const int t = threadIdx.x;
     for (int b = blockIdx.x; b < nbig; b += gridDim.x) {
         const int c = bs.big[b];
-        const int st = bs.start[c], n = bs.start[c + 1] - st;   // SMALL_CELL < n <= MAX_SORTED_CELL
+        const int st = bs.start[c], n = bs.start[c + 1] - st;   // n > SMALL_CELL
+        if (n > MAX_SORTED_CELL) continue;                       // summed in arrival order
         int P = 1;
         while (P < n) P <<= 1;
         for (int i = t; i < P; i += BIG_SORT_THREADS) s[i] = i < n ? bs.idx[st + i] : INT_MAX;
@@ -200,6 +205,63 @@ __global__ __launch_bounds__(BIG_SORT_THREADS) void k_order_big(BinState bs)
             }
         }
         for (int i = t; i < n; i += BIG_SORT_THREADS) bs.idx[st + i] = s[i];
+        __syncthreads();
+    }
+}
+
+// ---- deposit of the dense cells ---------------------------------------------------
+// One workgroup per dense cell: thread t sums its contiguous segment of the
+// cell's (id-ordered) particles into the 5 x 5 taps x 4 currents around the
+// cell; the 256 partial sums are combined by a butterfly within each wave and
+// then the four waves in a fixed order.  out[slot][tap][q], tap = 5 a + b for
+// the target cell (ix + a - 2, iy + b - 2), q = ro jx jy jz.
+constexpr int DENSE_VALUES = 25 * 4;
+template <typename T>
+__global__ __launch_bounds__(256) void k_dense_cells(double h, BinState bs)
+{
+    __shared__ T red[4][DENSE_VALUES];
+    const int nbig = min(*bs.nbig, bs.bigcap);
+    const DepRec<T> *__restrict__ rec = static_cast<const DepRec<T> *>(bs.rec);
+    T *__restrict__ out = static_cast<T *>(bs.dense);
+    const int t = threadIdx.x, lane = t % 64, wave = t / 64;
+    const T inv_h = T(1.0 / h);
+    for (int b = blockIdx.x; b < nbig; b += gridDim.x) {
+        const int c = bs.big[b];
+        const int st = bs.start[c], n = bs.start[c + 1] - st;
+        const int seg = (n + 255) / 256;
+        const int j0 = st + min(n, t * seg), j1 = st + min(n, (t + 1) * seg);
+        T acc[DENSE_VALUES];
+#pragma unroll
+        for (int k = 0; k < DENSE_VALUES; ++k) acc[k] = T(0);
+        for (int j = j0; j < j1; ++j) {
+            const DepRec<T> d = rec[bs.idx[j]];
+            T wx[5], wy[5];
+            weights4<T>(d.xl, wx);
+            weights4<T>(d.yl, wy);
+#pragma unroll
+            for (int a = 0; a < 5; ++a) {
+#pragma unroll
+                for (int bb = 0; bb < 5; ++bb) {
+                    const T w = (wx[a] * inv_h) * (wy[bb] * inv_h);
+                    acc[(5 * a + bb) * 4 + 0] += d.c0 * w;
+                    acc[(5 * a + bb) * 4 + 1] += d.c1 * w;
+                    acc[(5 * a + bb) * 4 + 2] += d.c2 * w;
+                    acc[(5 * a + bb) * 4 + 3] += d.c3 * w;
+                }
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < DENSE_VALUES; ++k) {
+#pragma unroll
+            for (int off = 32; off > 0; off >>= 1) acc[k] += __shfl_xor(acc[k], off);
+        }
+        if (lane == 0) {
+#pragma unroll
+            for (int k = 0; k < DENSE_VALUES; ++k) red[wave][k] = acc[k];
+        }
+        __syncthreads();
+        if (t < DENSE_VALUES)
+            out[(size_t)b * DENSE_VALUES + t] = (red[0][t] + red[1][t]) + (red[2][t] + red[3][t]);
         __syncthreads();
     }
 }
@@ -230,11 +292,12 @@ template <typename T> struct Acc<T, true> {
 // One block per 16 x 16 tile of output cells, one thread per output cell.
 // The block stages the records (written by the pass that binned each
 // particle) of its 20 x 20 halo cells' particles into LDS, CAP at a time, in
-// cell order; each output cell then sums, source row by source row (x
-// ascending), the particles of its 5 source cells in that row (cell order,
-// then particle id).  Work follows the local particle count, so clustered
-// plasma costs what it deposits.  No floating-point atomics: the result is
-// reproducible bit for bit.
+// cell order, leaving out the dense cells; each output cell then sums,
+// source row by source row (x ascending), the staged particles of its 5
+// source cells in that row (cell order, then particle id), and finally adds
+// the precomputed taps of the dense cells among its 25 source cells in
+// row-major order.  Work follows the local particle count.  No floating-point
+// atomics: the result is reproducible bit for bit.
 template <typename TP, typename TG, bool KAHAN>
 __global__ __launch_bounds__(256) void k_deposit_gather(
     int N, double h, const int *__restrict__ start, BinState bs, TG *__restrict__ ro,
@@ -243,8 +306,12 @@ __global__ __launch_bounds__(256) void k_deposit_gather(
 {
     using T = TP;   // arithmetic type of the tile
     constexpr int CAP = GatherCap<T>::value;
-    __shared__ int hs[DEP_HALO][DEP_HALO + 1];   // global list position of each halo cell
-    __shared__ int rowoff[DEP_HALO + 1];         // halo-list offset of each halo row
+    constexpr int H = DEP_HALO;
+    __shared__ int s_first[H * H];   // list position of each halo cell's particles
+    __shared__ int s_cnt[H * H];     // its staged particles (0 for a dense cell)
+    __shared__ int s_dsl[H * H];     // its dense slot, or -1
+    __shared__ int s_off[H * H + 1]; // staged offset of each halo cell (row-major)
+    __shared__ int sh[256];
     __shared__ T s_wx[5][CAP];
     __shared__ T s_wy[5][CAP];
     __shared__ T s_c[4][CAP];
@@ -254,40 +321,46 @@ __global__ __launch_bounds__(256) void k_deposit_gather(
     const int tpr = (N + DEP_TILE - 1) / DEP_TILE;
     const int X0 = (tile / tpr) * DEP_TILE, Y0 = (tile % tpr) * DEP_TILE;
     const DepRec<T> *__restrict__ rec = static_cast<const DepRec<T> *>(bs.rec);
+    const T *__restrict__ dense = static_cast<const T *>(bs.dense);
     const int hx0 = max(0, X0 - 2), hx1 = min(N, X0 + DEP_TILE + 2);
     const int hy0 = max(0, Y0 - 2), hy1 = min(N, Y0 + DEP_TILE + 2);
-    const int nrow = hx1 - hx0, ncol = hy1 - hy0;
+    const int nrow = hx1 - hx0, ncol = hy1 - hy0, nc = nrow * ncol;
     const int t = threadIdx.x;
 
-    for (int i = t; i < nrow * (ncol + 1); i += 256) {
-        const int r = i / (ncol + 1), c = i % (ncol + 1);
-        hs[r][c] = start[(hx0 + r) * N + hy0 + c];
+    for (int i = t; i < nc; i += 256) {
+        const int cell = (hx0 + i / ncol) * N + hy0 + i % ncol;
+        const int st = start[cell], ds = bs.dslot[cell];
+        s_first[i] = st;
+        s_dsl[i] = ds;
+        s_cnt[i] = ds >= 0 ? 0 : start[cell + 1] - st;
     }
     __syncthreads();
-    if (t == 0) {
-        int acc = 0;
-        for (int r = 0; r < nrow; ++r) {
-            rowoff[r] = acc;
-            acc += hs[r][ncol] - hs[r][0];
-        }
-        rowoff[nrow] = acc;
+    {
+        // Exclusive scan of the staged counts, two halo cells per thread.
+        const int i0 = 2 * t, i1 = 2 * t + 1;
+        const int c0 = i0 < nc ? s_cnt[i0] : 0, c1 = i1 < nc ? s_cnt[i1] : 0;
+        int total;
+        const int ex = block_exclusive_scan(c0 + c1, sh, total);
+        if (i0 < nc) s_off[i0] = ex;
+        if (i1 < nc) s_off[i1] = ex + c0;
+        if (t == 0) s_off[nc] = total;
     }
     __syncthreads();
-    const int total = rowoff[nrow];
+    const int total = s_off[nc];
 
-    // This thread's output cell and, per source row, its particle range.
+    // This thread's output cell and, per source row, its staged range.
     const int tx = t / DEP_TILE, ty = t % DEP_TILE;
     const int ox = X0 + tx, oy = Y0 + ty;
     const bool valid = ox < N && oy < N;
+    const int cl = max(oy - 2, hy0) - hy0, ch = min(oy + 3, hy1) - hy0;   // source columns
     int lo[5], hi[5];
 #pragma unroll
     for (int kr = 0; kr < 5; ++kr) {
         const int r = ox - 2 + kr - hx0;
         lo[kr] = hi[kr] = 0;
         if (valid && r >= 0 && r < nrow) {
-            const int c0 = max(oy - 2, hy0) - hy0, c1 = min(oy + 3, hy1) - hy0;
-            lo[kr] = rowoff[r] + hs[r][c0] - hs[r][0];
-            hi[kr] = rowoff[r] + hs[r][c1] - hs[r][0];
+            lo[kr] = s_off[r * ncol + cl];
+            hi[kr] = s_off[r * ncol + ch];
         }
     }
 
@@ -298,13 +371,13 @@ __global__ __launch_bounds__(256) void k_deposit_gather(
         // Stage: quartic weights (pre-divided by h) and current factors per
         // particle, from the records in cell order.
         for (int j = k0 + t; j < k1; j += 256) {
-            int lo_r = 0, hi_r = nrow;   // rowoff[lo_r] <= j < rowoff[hi_r]
-            while (hi_r - lo_r > 1) {
-                const int mid = (lo_r + hi_r) >> 1;
-                if (rowoff[mid] <= j) lo_r = mid;
-                else hi_r = mid;
+            int lo_c = 0, hi_c = nc;   // s_off[lo_c] <= j < s_off[hi_c]
+            while (hi_c - lo_c > 1) {
+                const int mid = (lo_c + hi_c) >> 1;
+                if (s_off[mid] <= j) lo_c = mid;
+                else hi_c = mid;
             }
-            const DepRec<T> d = rec[bs.idx[hs[lo_r][0] + (j - rowoff[lo_r])]];
+            const DepRec<T> d = rec[bs.idx[s_first[lo_c] + (j - s_off[lo_c])]];
             T w[5];
             weights4<T>(d.xl, w);
 #pragma unroll
@@ -333,6 +406,23 @@ __global__ __launch_bounds__(256) void k_deposit_gather(
             }
         }
         __syncthreads();
+    }
+    // Dense source cells: their taps toward this cell.
+    if (valid) {
+        for (int kr = 0; kr < 5; ++kr) {
+            const int r = ox - 2 + kr - hx0;
+            if (r < 0 || r >= nrow) continue;
+            for (int c = cl; c < ch; ++c) {
+                const int ds = s_dsl[r * ncol + c];
+                if (ds < 0) continue;
+                const int tap = 5 * (ox - (hx0 + r) + 2) + (oy - (hy0 + c) + 2);
+                const T *v = dense + (size_t)ds * DENSE_VALUES + 4 * tap;
+                a_ro.add(v[0]);
+                a_jx.add(v[1]);
+                a_jy.add(v[2]);
+                a_jz.add(v[3]);
+            }
+        }
     }
 
     if (valid) {
@@ -374,11 +464,13 @@ hipError_t launch_deposit(const PushGeom &g, const SortArrays<TP> &a, bool use_f
         k_scatter<<<pblocks, 256, 0, s>>>(g.n, ncell, bs);
         k_order_cells<<<(ncell + 255) / 256, 256, 0, s>>>(ncell, bs);
         k_order_big<<<256, BIG_SORT_THREADS, 0, s>>>(bs);
+        k_dense_cells<TP><<<512, 256, 0, s>>>(g.h, bs);
     } else {
-        // Empty sort: every cell list is empty.
+        // Empty sort: every cell list is empty (and no cell is dense).
         const int nb = (ncell + 1 + SCAN_TILE - 1) / SCAN_TILE;
         k_scan_blocks<<<nb, SCAN_THREADS, 0, s>>>(ncell, bs.count, bs.blocks);
         k_scan_final<<<nb, SCAN_THREADS, 0, s>>>(ncell, nb, bs.count, bs.blocks, bs.start);
+        k_order_cells<<<(ncell + 255) / 256, 256, 0, s>>>(ncell, bs);
     }
     const int tpr = (N + DEP_TILE - 1) / DEP_TILE;
     const int tiles = tpr * tpr;

@@ -46,9 +46,11 @@ struct BinState {
     unsigned *err;    // sticky error bits
     void *rec;        // per particle: deposit records (deposit.hip)
     int64_t cap;      // particles the per-particle arrays hold
-    int *big;         // cells whose list is sorted by a whole workgroup
+    int *big;         // dense cells (more than 64 particles), in queue order
     int *nbig;        // how many cells `big` holds
     int bigcap;       // capacity of `big`
+    int *dslot;       // N*N: a cell's slot in `big`, or -1
+    void *dense;      // per dense cell: its 5 x 5 x 4 deposit (deposit.hip)
 };
 
 // What the deposit needs of one binned plasma particle (weights.py:35-67):

@@ -42,6 +42,7 @@ inline const char *hipGetErrorString(hipError_t) { return "emu"; }
 extern thread_local dim3 threadIdx, blockIdx;
 extern dim3 gridDim, blockDim;
 extern thread_local double emu_shfl_d[1024];
+extern thread_local float emu_shfl_f[1024];
 void emu_sync_block();
 void emu_sync_wave();
 inline void __syncthreads() { emu_sync_block(); }
@@ -50,6 +51,14 @@ inline double __shfl_xor(double v, int off) {
     emu_shfl_d[t] = v;
     emu_sync_wave();
     double r = emu_shfl_d[(t % 64 ^ off) + w * 64];
+    emu_sync_wave();
+    return r;
+}
+inline float __shfl_xor(float v, int off) {
+    int t = threadIdx.x, w = t / 64;
+    emu_shfl_f[t] = v;
+    emu_sync_wave();
+    float r = emu_shfl_f[(t % 64 ^ off) + w * 64];
     emu_sync_wave();
     return r;
 }
