@@ -377,10 +377,10 @@ int lc3d_create(const lc3d_params *p, lc3d_ctx **out)
               hipMalloc(&c->idx, cap * sizeof(int)) == hipSuccess &&
               hipMalloc(&c->err, sizeof(unsigned)) == hipSuccess &&
               hipMalloc(&c->rec, cap * deposit_record_bytes(f32)) == hipSuccess &&
-              hipMalloc(&c->big, (cap / 64 + 2) * sizeof(int)) == hipSuccess &&
+              hipMalloc(&c->big, dense_cell_capacity(cap) * sizeof(int)) == hipSuccess &&
               hipMalloc(&c->nbig, sizeof(int)) == hipSuccess &&
               hipMalloc(&c->dslot, nn * sizeof(int)) == hipSuccess &&
-              hipMalloc(&c->dense, (cap / 64 + 2) * 100 * tg) == hipSuccess &&
+              hipMalloc(&c->dense, dense_cell_capacity(cap) * 100 * tg) == hipSuccess &&
               hipMalloc(&c->work, 7 * nn * tg) == hipSuccess &&
               hipMalloc(&c->rowsum, N * sizeof(double)) == hipSuccess &&
               hipMalloc(&c->lam, N * sizeof(double)) == hipSuccess;
@@ -445,7 +445,7 @@ int lc3d_create(const lc3d_params *p, lc3d_ctx **out)
     if (e == hipSuccess) e = hipMemset(c->count, 0, (nn + 1) * sizeof(int));
     if (e == hipSuccess) e = hipMemset(c->err, 0, sizeof(unsigned));
     if (e == hipSuccess) e = hipMemset(c->nbig, 0, sizeof(int));
-    c->bigcap = (int)(cap / 64 + 2);
+    c->bigcap = dense_cell_capacity(cap);
     if (e == hipSuccess) e = hipDeviceSynchronize();
     if (e != hipSuccess) {
         lc3d_destroy(c);

@@ -32,7 +32,7 @@ constexpr int DEP_HALO = DEP_TILE + 4;
 constexpr int SCAN_THREADS = 256;
 constexpr int SCAN_PER = 8;
 constexpr int SCAN_TILE = SCAN_THREADS * SCAN_PER;
-constexpr int SMALL_CELL = 64;          // lists sorted by their cell's thread
+constexpr int SMALL_CELL = 16;          // lists sorted by their cell's thread
 constexpr int MAX_SORTED_CELL = 16384;  // lists sorted by a workgroup in LDS
 
 // ---- binning ------------------------------------------------------------------

@@ -46,7 +46,7 @@ struct BinState {
     unsigned *err;    // sticky error bits
     void *rec;        // per particle: deposit records (deposit.hip)
     int64_t cap;      // particles the per-particle arrays hold
-    int *big;         // dense cells (more than 64 particles), in queue order
+    int *big;         // dense cells (more than 16 particles), in queue order
     int *nbig;        // how many cells `big` holds
     int bigcap;       // capacity of `big`
     int *dslot;       // N*N: a cell's slot in `big`, or -1
@@ -61,6 +61,10 @@ template <typename T> struct alignas(16) DepRec {
     T xl, yl, c0, c1, c2, c3;
     int iy;
 };
+
+// Capacity of the dense-cell queue: cells with more than 16 of the at most
+// `cap` particles.
+inline int dense_cell_capacity(int64_t cap) { return (int)(cap / 16 + 2); }
 
 inline size_t deposit_record_bytes(bool fp32_bulk)
 {

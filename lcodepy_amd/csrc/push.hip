@@ -46,13 +46,12 @@ __device__ __forceinline__ bool reflect(double x_init, double y_init, double b, 
     return hit;
 }
 
+// Pushes particle k; with BIN returns its deposit cell (or -1) after writing
+// its deposit record, otherwise -1.
 template <typename TP, typename TF, bool FIRST, bool BIN>
-__global__ __launch_bounds__(256) void k_push(PushGeom g, Fields6<TF> f, SortArrays<TP> a,
-                                              BinState bs, const int *__restrict__ order)
+__device__ __forceinline__ int push_one(const PushGeom &g, const Fields6<TF> &f,
+                                        const SortArrays<TP> &a, const BinState &bs, int64_t k)
 {
-    const int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    if (j >= g.n) return;
-    const int64_t k = order ? (int64_t)order[j] : j;   // the particle this thread pushes
     const int N = g.N;
     const double b = g.boundary, dxi = g.xi_step;
 
@@ -91,7 +90,7 @@ __global__ __launch_bounds__(256) void k_push(PushGeom g, Fields6<TF> f, SortArr
             a.q[k] = q;
         }
         if (BIN) bs.key[k] = -1;
-        return;
+        return -1;
     }
 
     // Midpoint between the previous position and the current estimate.
@@ -188,17 +187,49 @@ __global__ __launch_bounds__(256) void k_push(PushGeom g, Fields6<TF> f, SortArr
     a.pz[k] = pz;
     if (FIRST || absorbed) a.q[k] = absorbed ? TP(0) : q;
 
+    int key = -1;
     if (BIN) {
-        int key = -1;
-        if (!absorbed) {
+        if (!absorbed)
             key = bin_particle<TP>(xi0 + xo, yi0 + yo, g.h, N, (double)px, (double)py,
                                    (double)pz, m, (double)q, static_cast<DepRec<TP> *>(bs.rec),
                                    k, bs.err);
-            if (key >= 0) atomicAdd(&bs.count[key], 1);
-        }
         bs.key[k] = key;
     }
+    return key;
 }
+
+// Adds the particles of the wave to their cells' counts: consecutive lanes
+// with the same cell (the common case when particles are pushed in cell
+// order) add their number with one atomic.  Every lane of the wave calls it;
+// key < 0 adds nothing.
+__device__ __forceinline__ void count_keys(int *count, int key)
+{
+    const int lane = threadIdx.x & 63;
+    const int prev = __shfl_up(key, 1);
+    const bool head = key >= 0 && (lane == 0 || prev != key);
+    const unsigned long long ends = __ballot(head || key < 0);   // lanes that start a new run
+    if (head) {
+        const unsigned long long above = ends & ~((2ull << lane) - 1ull);
+        const int next = above ? __builtin_ctzll(above) : 64;
+        atomicAdd(&count[key], next - lane);
+    }
+}
+
+template <typename TP, typename TF, bool FIRST, bool BIN>
+__global__ __launch_bounds__(256) void k_push(PushGeom g, Fields6<TF> f, SortArrays<TP> a,
+                                              BinState bs, const int *__restrict__ order)
+{
+    // Threads past the end stay to the count so that whole waves reach it.
+    const int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    int key = -1;
+    if (j < g.n) {
+        const int64_t k = order ? (int64_t)order[j] : j;   // the particle this thread pushes
+        key = push_one<TP, TF, FIRST, BIN>(g, f, a, bs, k);
+    }
+    if (BIN) count_keys(bs.count, key);
+}
+
+
 
 template <typename TP>
 hipError_t launch_move_wo_fields(int64_t n, double xi_step, const TP *m, double *xo, double *yo,

@@ -19,6 +19,7 @@ thread_local dim3 threadIdx, blockIdx;
 dim3 gridDim, blockDim;
 thread_local double emu_shfl_d[1024];
 thread_local float emu_shfl_f[1024];
+thread_local int emu_shfl_i[1024];
 thread_local unsigned char *emu_lds = nullptr;
 
 namespace {

@@ -43,6 +43,7 @@ extern thread_local dim3 threadIdx, blockIdx;
 extern dim3 gridDim, blockDim;
 extern thread_local double emu_shfl_d[1024];
 extern thread_local float emu_shfl_f[1024];
+extern thread_local int emu_shfl_i[1024];
 void emu_sync_block();
 void emu_sync_wave();
 inline void __syncthreads() { emu_sync_block(); }
@@ -53,6 +54,24 @@ inline double __shfl_xor(double v, int off) {
     double r = emu_shfl_d[(t % 64 ^ off) + w * 64];
     emu_sync_wave();
     return r;
+}
+inline int __shfl_up(int v, unsigned d) {
+    int t = threadIdx.x;
+    emu_shfl_i[t] = v;
+    emu_sync_wave();
+    int r = (unsigned)(t % 64) >= d ? emu_shfl_i[t - d] : v;
+    emu_sync_wave();
+    return r;
+}
+inline unsigned long long __ballot(int pred) {
+    int t = threadIdx.x, w = t / 64;
+    emu_shfl_i[t] = pred != 0;
+    emu_sync_wave();
+    unsigned long long m = 0;
+    for (int l = 0; l < 64 && w * 64 + l < (int)(blockDim.x * blockDim.y * blockDim.z); ++l)
+        if (emu_shfl_i[w * 64 + l]) m |= 1ull << l;
+    emu_sync_wave();
+    return m;
 }
 inline float __shfl_xor(float v, int off) {
     int t = threadIdx.x, w = t / 64;
