@@ -3,7 +3,9 @@
 
 Workload (the 513 x 513, 4 ppc configuration): transverse window 25.6
 (step 0.05, N = 513), 4 plasma macro-particles per cell (964^2 = 929,296)
-with seeded thermal momenta ~ N(0, 1e-3), background ions, xi_step 0.02 over
+with seeded thermal momenta p / (m c) ~ N(0, 1e-3) per component (a macro-
+particle stores p scaled by its mass m = h^2 / ppc, gamma = sqrt(m^2 + p^2) /
+m, so the stored momenta are m times the draws), background ions, xi_step 0.02 over
 a window of length 20 (1001 xi steps); a 1e6-particle seeded electron
 driver, n_b/n0 = 1, sigma_r = 0.5, sigma_xi = 1 (truncated at +-3 sigma),
 gamma ~ N(1000, 10).  One "step" is one xi step of the whole path: beam layer
@@ -35,7 +37,8 @@ METRIC = ('xi-steps/s at 513² 4ppc, FP32-bulk vs FP64 @Ez err≤1e-4; HBM TB/s'
 HBM_PEAK = 8.0e12
 MFMA_PEAK = {'fp32-bulk': 157.3e12, 'fp64': 78.6e12}   # dense matrix-core FLOP/s
 WINDOW_STEPS = 1001                                     # xi steps of one time step
-TRAFFIC_FILE = os.path.join(ROOT, 'profiles', 'solve_gemm_513_4ppc_hbm_traffic.md')
+TRAFFIC_FILE = os.path.join(ROOT, 'profiles', 'xi_step_513_4ppc_hbm_traffic.md')
+LAUNCHES_PER_STEP = {'push': 3, 'deposit_gather': 3, 'solve_gemm': 8}
 
 
 def workload_config(precision: str) -> dict:
@@ -69,14 +72,25 @@ def driver_beam(n_particles: int, xi_step: float, seed: int = 1) -> np.ndarray:
     return b[np.argsort(-b['xi'])]
 
 
-def thermalize(particles: dict, seed: int = 2, sigma: float = 1e-3) -> None:
-    import torch
+def electron_mass(cfg) -> float:
+    """Mass of a plasma electron macro-particle: h^2 / ppc (electron mass 1)."""
+    return cfg.getfloat('transverse-step') ** 2 / cfg.getfloat('plasma-particles-per-cell')
+
+
+def thermal_momenta(m: float, shape, seed: int = 2, sigma: float = 1e-3) -> dict:
+    """Seeded thermal momenta of macro-particles of mass m: p = m u with
+    u ~ N(0, sigma) per component (sigma in units of m c)."""
     rng = np.random.default_rng(seed)
-    for arrs in particles.values():
-        for k in ('px', 'py', 'pz'):
-            t = getattr(arrs, k)
-            noise = rng.normal(0.0, sigma, tuple(t.shape))
-            t.copy_(torch.as_tensor(noise, dtype=t.dtype, device=t.device))
+    return {k: m * rng.normal(0.0, sigma, shape) for k in ('px', 'py', 'pz')}
+
+
+def thermalize(particles: dict, m: float, seed: int = 2, sigma: float = 1e-3) -> None:
+    """Draws the electrons' momenta (the only mobile sort of the workload)."""
+    import torch
+    arrs = particles['electrons']
+    for k, p in thermal_momenta(m, tuple(arrs.m.shape), seed, sigma).items():
+        t = getattr(arrs, k)
+        t.copy_(torch.as_tensor(p, dtype=t.dtype, device=t.device))
 
 
 class Run:
@@ -92,7 +106,7 @@ class Run:
         self.cfg = cfg
         self.N = cfg.getint('window-width-steps')
         fields, particles, currents, const, xi0 = init_plasma(cfg)
-        thermalize(particles)
+        thermalize(particles, electron_mass(cfg))
         self.n_plasma = sum(int(a.m.numel()) for a in particles.values())
         self.solver = PusherAndSolver3D(cfg)
         self.solver.verbose = False
@@ -136,26 +150,38 @@ def gemm_flops(N: int) -> float:
     return 7 * 2.0 * N ** 3
 
 
-def hbm_traffic():
-    """HBM bytes per solve-GEMM launch from the rocprofv3 FETCH_SIZE and
-    WRITE_SIZE passes (scaled by a copy of known size), when summarised."""
+def hbm_traffic(precision: str, kernel_class: str):
+    """HBM bytes per launch of a kernel class from the rocprofv3 FETCH_SIZE
+    and WRITE_SIZE passes (scaled by a copy of known size), when summarised."""
+    key = f'traffic_bytes_per_launch {precision} {kernel_class}:'
     try:
         with open(TRAFFIC_FILE) as f:
             for line in f:
-                if line.startswith('traffic_bytes_per_launch:'):
+                if line.startswith(key):
                     return float(line.split(':', 1)[1])
     except OSError:
         pass
     return None
 
 
-def push_bytes(n: int, N: int, bulk: int) -> int:
-    """Algorithmic bytes of one smart-push launch (plasma3d/move.py:52-148):
-    reads x_init y_init x_offt_prev y_offt_prev x_offt y_offt (float64) and
-    px py pz (prev) q m (bulk type), writes x_offt y_offt (float64) and
-    px py pz (bulk type); plus one read of the six (N, N) fields."""
-    fbytes = 8 if bulk == 8 else 4
-    return n * (6 * 8 + 5 * bulk + 2 * 8 + 3 * bulk) + 6 * N * N * fbytes
+def deposit_bytes(n: int, N: int, bulk: int) -> int:
+    """Algorithmic bytes of one deposit (weights.py:35-67): the nine particle
+    arrays read (x_init y_init x_offt y_offt pz float64, px py q m of the
+    bulk type) and the four current grids written."""
+    return n * (5 * 8 + 4 * bulk) + 4 * N * N * bulk
+
+
+def push_bytes(n: int, N: int, bulk: int) -> float:
+    """Algorithmic bytes of one smart-push launch (plasma3d/move.py:52-148),
+    averaged over the three pushes of an xi step.  Pushes 2 and 3 read
+    x_init y_init x_offt_prev y_offt_prev x_offt y_offt pz_prev (float64) and
+    px_prev py_prev q m (bulk type) and write x_offt y_offt pz (float64) and
+    px py (bulk type), and read the six half-step fields (bulk type); push 1
+    (with the free-streaming estimate) reads the previous layer only (4
+    float64 + pz + 4 bulk), writes also q, and reads the float64 fields."""
+    later = n * (7 * 8 + 4 * bulk + 3 * 8 + 2 * bulk) + 6 * N * N * bulk
+    first = n * (5 * 8 + 4 * bulk + 3 * 8 + 3 * bulk) + 6 * N * N * 8
+    return (first + 2 * later) / 3
 
 
 def cpu_baseline(sample_steps: int) -> dict:
@@ -169,9 +195,8 @@ def cpu_baseline(sample_steps: int) -> dict:
     cores = os.cpu_count() or 1
     p = O.Params.from_config(cfg, nthreads=cores)
     fields, particles, currents, const, _ = O.init_plasma(p)
-    rng = np.random.default_rng(2)
-    for k in ('px', 'py', 'pz'):
-        particles['electrons'][k] = rng.normal(0.0, 1e-3, particles['electrons'][k].shape)
+    e = particles['electrons']
+    e.update(thermal_momenta(electron_mass(cfg), e['m'].shape))
     rho_prev = np.zeros((p.N, p.N))
     grid = (np.arange(p.N) - p.N // 2) * p.h
     rho = -np.exp(-(grid[:, None] ** 2 + grid[None, :] ** 2) / (2 * 0.25))
@@ -220,8 +245,9 @@ def main() -> int:
                          'the reference dispatch for rocprofv3 FETCH_SIZE/WRITE_SIZE')
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get('WORLD_SIZE', '1')) != 1:
-        print('bench.py measures one GPU: this build has no multi-GPU path '
-              '(independent runs would shard by run; see DESIGN.md)', file=sys.stderr)
+        print('bench.py measures one GPU only: the xi step has no multi-GPU path here '
+              '(a window is one GPU\'s work; independent time steps or runs are the unit '
+              'that spreads over GPUs)', file=sys.stderr)
         return 2
     timing_steps = args.timing_steps
     if timing_steps is None:
@@ -261,9 +287,26 @@ def main() -> int:
     value = args.steps / r['elapsed']
     N, n = r['N'], r['n']
     bulk = 4 if head == 'fp32-bulk' else 8
-    t_gemm, t_push, t_dep = (r['kt'][k][0] for k in ('solve_gemm', 'push', 'deposit_gather'))
-    fl = gemm_flops(N)
-    pb = push_bytes(n, N, bulk)
+    kt = {k: v[0] for k, v in r['kt'].items()}
+    per_step = {k: kt[k] * LAUNCHES_PER_STEP[k] for k in kt}
+    dominant = max(per_step, key=per_step.get)
+
+    def entry(k):
+        if k == 'solve_gemm':
+            fl = gemm_flops(N)
+            return {'bound': 'mfma', 'kernel': 'k_gemm_mfma (solve transform pass, 7 fields)',
+                    'achieved': fl / kt[k] / 1e12, 'peak': MFMA_PEAK[head] / 1e12,
+                    'unit': 'TFLOP/s', 'frac': fl / kt[k] / MFMA_PEAK[head],
+                    'traffic': hbm_traffic(head, k), 'seconds_per_launch': kt[k],
+                    'seconds_per_xi_step': per_step[k]}
+        nbytes = push_bytes(n, N, bulk) if k == 'push' else deposit_bytes(n, N, bulk)
+        name = 'k_push (smart push, 3 per xi step)' if k == 'push' else \
+            'k_deposit_gather (plasma deposit, 3 per xi step)'
+        return {'bound': 'hbm', 'kernel': name, 'achieved': nbytes / kt[k] / 1e9,
+                'peak': HBM_PEAK / 1e9, 'unit': 'GB/s', 'frac': nbytes / kt[k] / HBM_PEAK,
+                'traffic': hbm_traffic(head, k), 'seconds_per_launch': kt[k],
+                'seconds_per_xi_step': per_step[k]}
+
     line = {
         'metric': METRIC, 'value': value, 'unit': 'xi-steps/s', 'n_gpus': 1,
         'steps': args.steps, 'warmup': args.warmup,
@@ -278,16 +321,10 @@ def main() -> int:
                    'precision': head, 'N': N, 'plasma_particles': n,
                    'beam_particles': args.beam_particles},
         'particle_pushes_per_s': 3 * n * value,
-        'kernel_avg_s': {k: v[0] for k, v in r['kt'].items()},
+        'kernel_avg_s': kt,
         'kernel_launches_timed': {k: v[1] for k, v in r['kt'].items()},
-        'roofline': {'bound': 'mfma', 'kernel': 'k_gemm_mfma (solve transform pass, 7 fields)',
-                     'achieved': fl / t_gemm / 1e12, 'peak': MFMA_PEAK[head] / 1e12,
-                     'unit': 'TFLOP/s', 'frac': fl / t_gemm / MFMA_PEAK[head],
-                     'traffic': hbm_traffic()},
-        'roofline_hbm': {
-            'k_push': {'achieved': pb / t_push / 1e9, 'peak': HBM_PEAK / 1e9, 'unit': 'GB/s',
-                       'frac': pb / t_push / HBM_PEAK},
-        },
+        'roofline': entry(dominant),
+        'roofline_all': {k: entry(k) for k in kt},
     }
     if 'fp64' in results and head != 'fp64':
         r64 = results['fp64']
