@@ -22,7 +22,7 @@ struct lc3d_ctx {
     // binning
     int *count = nullptr, *start = nullptr, *blocks = nullptr, *key = nullptr, *idx = nullptr;
     unsigned *err = nullptr;
-    void *rec = nullptr;
+    void *rec = nullptr, *rec_sorted = nullptr;
     int *big = nullptr, *nbig = nullptr, *dslot = nullptr;
     void *dense = nullptr;
     int bigcap = 0;
@@ -109,7 +109,7 @@ BinState bin_state(const lc3d_ctx *c)
 {
     return BinState{c->count, c->start, c->blocks, c->key, c->idx, c->err, c->rec,
                     std::max<int64_t>(1, c->cap), c->big, c->nbig, c->bigcap, c->dslot,
-                    c->dense};
+                    c->dense, c->rec_sorted};
 }
 
 template <typename TP> SortArrays<TP> sort_arrays(const lc3d_sort *s)
@@ -377,6 +377,7 @@ int lc3d_create(const lc3d_params *p, lc3d_ctx **out)
               hipMalloc(&c->idx, cap * sizeof(int)) == hipSuccess &&
               hipMalloc(&c->err, sizeof(unsigned)) == hipSuccess &&
               hipMalloc(&c->rec, cap * deposit_record_bytes(f32)) == hipSuccess &&
+              hipMalloc(&c->rec_sorted, cap * deposit_record_bytes(f32)) == hipSuccess &&
               hipMalloc(&c->big, dense_cell_capacity(cap) * sizeof(int)) == hipSuccess &&
               hipMalloc(&c->nbig, sizeof(int)) == hipSuccess &&
               hipMalloc(&c->dslot, nn * sizeof(int)) == hipSuccess &&
@@ -469,6 +470,7 @@ void lc3d_destroy(lc3d_ctx *c)
     (void)hipFree(c->idx);
     (void)hipFree(c->err);
     (void)hipFree(c->rec);
+    (void)hipFree(c->rec_sorted);
     (void)hipFree(c->big);
     (void)hipFree(c->nbig);
     (void)hipFree(c->dslot);

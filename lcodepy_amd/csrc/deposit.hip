@@ -11,11 +11,12 @@
 //               are counted back down to zero, ready for the next deposit);
 //   4. order    each cell's list sorted by particle id, so the summation
 //               order below does not depend on atomic arrival order;
-//   5. gather   one workgroup per 16x16 tile of output cells stages the
-//               particles of the 20x20 source cells around it in LDS
-//               (quartic weights / h and the four charges per particle),
-//               then each thread sums, for its own output cell, the
-//               contributions of the 5 source rows in ascending order.
+//   5. taps     per dense cell (more than 16 particles) its 5 x 5 x 4 taps,
+//               by one workgroup with a fixed-order reduction;
+//   6. gather   one workgroup per 16x16 tile of output cells forms, in five
+//               passes over the x offset, the taps of the 20x20 source
+//               cells around it in LDS; each thread adds, for its own output
+//               cell, the 25 taps that reach it in a fixed order.
 //
 // Every output cell is written by exactly one thread: no floating-point
 // atomics, and the result is bit-reproducible run to run.  In the FP32-bulk
@@ -24,6 +25,7 @@
 #include "kernels.h"
 
 #include <climits>
+#include <type_traits>
 
 namespace lc3d {
 
@@ -160,7 +162,7 @@ __global__ __launch_bounds__(256) void k_order_cells(int ncell, BinState bs)
         if (e - s > MAX_SORTED_CELL) atomicOr(bs.err, ERR_CELL_OVERFLOW);
     }
     bs.dslot[c] = slot;
-    if (slot >= 0 || e - s < 2) return;
+    if (slot >= 0) return;
     int *a = bs.idx;
     for (int i = s + 1; i < e; ++i) {
         const int v = a[i];
@@ -170,6 +172,23 @@ __global__ __launch_bounds__(256) void k_order_cells(int ncell, BinState bs)
             --j;
         }
         a[j + 1] = v;
+    }
+}
+
+// The deposit records in list order: rec_sorted[i] = rec[idx[i]] for every
+// binned list position i < start[ncell], one thread per 16-byte word of a
+// record (a record is rec_words words), so the writes are contiguous.  The
+// gather reads the non-dense cells' records from here.
+__global__ __launch_bounds__(256) void k_sort_records(int ncell, BinState bs, int rec_words)
+{
+    const int64_t nwords = (int64_t)bs.start[ncell] * rec_words;
+    const int4 *__restrict__ src = static_cast<const int4 *>(bs.rec);
+    int4 *__restrict__ dst = static_cast<int4 *>(bs.rec_sorted);
+    for (int64_t o = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; o < nwords;
+         o += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t i = o / rec_words;
+        const int w = (int)(o - i * rec_words);
+        dst[o] = src[(int64_t)bs.idx[i] * rec_words + w];
     }
 }
 
@@ -267,11 +286,6 @@ __global__ __launch_bounds__(256) void k_dense_cells(double h, BinState bs)
 }
 
 // ---- gather deposit -------------------------------------------------------------------
-// Particles staged per LDS chunk.
-template <typename T> struct GatherCap;
-template <> struct GatherCap<double> { static constexpr int value = 256; };
-template <> struct GatherCap<float> { static constexpr int value = 512; };
-
 template <typename T, bool KAHAN> struct Acc;
 template <typename T> struct Acc<T, false> {
     T s = 0;
@@ -289,38 +303,82 @@ template <typename T> struct Acc<T, true> {
     __device__ __forceinline__ T get() const { return s; }
 };
 
-// One block per 16 x 16 tile of output cells, one thread per output cell.
-// The block stages the records (written by the pass that binned each
-// particle) of its 20 x 20 halo cells' particles into LDS, CAP at a time, in
-// cell order, leaving out the dense cells; each output cell then sums,
-// source row by source row (x ascending), the staged particles of its 5
-// source cells in that row (cell order, then particle id), and finally adds
-// the precomputed taps of the dense cells among its 25 source cells in
-// row-major order.  Work follows the local particle count.  No floating-point
-// atomics: the result is reproducible bit for bit.
+template <typename T> struct alignas(4 * sizeof(T)) Tap4 {
+    T v[4];
+};
+
+// The five taps toward x offset A - 2 (target cells (ix + A - 2, iy + b - 2),
+// b = 0..4) of one source cell, for the four currents: the sum over its
+// particles, in list order, of c_q (w_A(xl) / h) (w_b(yl) / h), from the
+// records in list order (rec = rec_sorted, first = the cell's list start),
+// or the precomputed block of a dense cell.
+template <int A, typename T>
+__device__ __forceinline__ void cell_taps(const DepRec<T> *__restrict__ rec,
+                                          const T *__restrict__ dense, int first, int n, int dsl,
+                                          T inv_h, Tap4<T> tp[5])
+{
+#pragma unroll
+    for (int b = 0; b < 5; ++b)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) tp[b].v[q] = T(0);
+    if (dsl >= 0) {
+        const T *v = dense + (size_t)dsl * DENSE_VALUES + 4 * (5 * A);
+#pragma unroll
+        for (int b = 0; b < 5; ++b)
+#pragma unroll
+            for (int q = 0; q < 4; ++q) tp[b].v[q] = v[4 * b + q];
+        return;
+    }
+    // Up to four records in flight at a time.
+    for (int j0 = 0; j0 < n; j0 += 4) {
+        DepRec<T> d[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+            if (j0 + k < n) d[k] = rec[first + j0 + k];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            if (j0 + k >= n) break;
+            const T wx = w4_tap<A>(d[k].xl) * inv_h;
+            T wy[5];
+            weights4<T>(d[k].yl, wy);
+#pragma unroll
+            for (int b = 0; b < 5; ++b) {
+                const T w = wx * (wy[b] * inv_h);
+                tp[b].v[0] += d[k].c0 * w;
+                tp[b].v[1] += d[k].c1 * w;
+                tp[b].v[2] += d[k].c2 * w;
+                tp[b].v[3] += d[k].c3 * w;
+            }
+        }
+    }
+}
+
+// One block per 16 x 16 tile of output cells, one thread per output cell,
+// in five passes over the x offset A of the source cells (source row
+// ox - (A - 2)).  In a pass every cell of the tile's 20 x 20 halo forms its
+// five taps toward that offset (cell_taps: from the deposit records its
+// particles' binning pass wrote, in list order, or a dense cell's
+// precomputed block) into LDS; then each output cell adds the five taps that
+// reach it (source column oy - (b - 2), b ascending).  Every output cell is
+// thus the same fixed-order sum of 25 taps, written by exactly one thread:
+// no floating-point atomics, reproducible bit for bit.
 template <typename TP, typename TG, bool KAHAN>
 __global__ __launch_bounds__(256) void k_deposit_gather(
     int N, double h, const int *__restrict__ start, BinState bs, TG *__restrict__ ro,
     TG *__restrict__ jx, TG *__restrict__ jy, TG *__restrict__ jz, const TG *__restrict__ bg_ro,
     TG *__restrict__ ro1, TG *__restrict__ jx1, TG *__restrict__ jy1, TG *__restrict__ jz1)
 {
-    using T = TP;   // arithmetic type of the tile
-    constexpr int CAP = GatherCap<T>::value;
+    using T = TP;   // arithmetic type of the taps
     constexpr int H = DEP_HALO;
     __shared__ int s_first[H * H];   // list position of each halo cell's particles
-    __shared__ int s_cnt[H * H];     // its staged particles (0 for a dense cell)
+    __shared__ int s_cnt[H * H];     // their number
     __shared__ int s_dsl[H * H];     // its dense slot, or -1
-    __shared__ int s_off[H * H + 1]; // staged offset of each halo cell (row-major)
-    __shared__ int sh[256];
-    __shared__ T s_wx[5][CAP];
-    __shared__ T s_wy[5][CAP];
-    __shared__ T s_c[4][CAP];
-    __shared__ int s_cy[CAP];
+    __shared__ Tap4<T> s_tap[5][H * H];
 
     const int tile = xcd_swizzle(blockIdx.x, gridDim.x);
     const int tpr = (N + DEP_TILE - 1) / DEP_TILE;
     const int X0 = (tile / tpr) * DEP_TILE, Y0 = (tile % tpr) * DEP_TILE;
-    const DepRec<T> *__restrict__ rec = static_cast<const DepRec<T> *>(bs.rec);
+    const DepRec<T> *__restrict__ rec = static_cast<const DepRec<T> *>(bs.rec_sorted);
     const T *__restrict__ dense = static_cast<const T *>(bs.dense);
     const int hx0 = max(0, X0 - 2), hx1 = min(N, X0 + DEP_TILE + 2);
     const int hy0 = max(0, Y0 - 2), hy1 = min(N, Y0 + DEP_TILE + 2);
@@ -329,108 +387,54 @@ __global__ __launch_bounds__(256) void k_deposit_gather(
 
     for (int i = t; i < nc; i += 256) {
         const int cell = (hx0 + i / ncol) * N + hy0 + i % ncol;
-        const int st = start[cell], ds = bs.dslot[cell];
+        const int st = start[cell];
         s_first[i] = st;
-        s_dsl[i] = ds;
-        s_cnt[i] = ds >= 0 ? 0 : start[cell + 1] - st;
+        s_cnt[i] = start[cell + 1] - st;
+        s_dsl[i] = bs.dslot[cell];
     }
     __syncthreads();
-    {
-        // Exclusive scan of the staged counts, two halo cells per thread.
-        const int i0 = 2 * t, i1 = 2 * t + 1;
-        const int c0 = i0 < nc ? s_cnt[i0] : 0, c1 = i1 < nc ? s_cnt[i1] : 0;
-        int total;
-        const int ex = block_exclusive_scan(c0 + c1, sh, total);
-        if (i0 < nc) s_off[i0] = ex;
-        if (i1 < nc) s_off[i1] = ex + c0;
-        if (t == 0) s_off[nc] = total;
-    }
-    __syncthreads();
-    const int total = s_off[nc];
 
-    // This thread's output cell and, per source row, its staged range.
     const int tx = t / DEP_TILE, ty = t % DEP_TILE;
     const int ox = X0 + tx, oy = Y0 + ty;
     const bool valid = ox < N && oy < N;
-    const int cl = max(oy - 2, hy0) - hy0, ch = min(oy + 3, hy1) - hy0;   // source columns
-    int lo[5], hi[5];
-#pragma unroll
-    for (int kr = 0; kr < 5; ++kr) {
-        const int r = ox - 2 + kr - hx0;
-        lo[kr] = hi[kr] = 0;
-        if (valid && r >= 0 && r < nrow) {
-            lo[kr] = s_off[r * ncol + cl];
-            hi[kr] = s_off[r * ncol + ch];
-        }
-    }
-
-    Acc<T, KAHAN> a_ro, a_jx, a_jy, a_jz;
     const T inv_h = T(1.0 / h);
-    for (int k0 = 0; k0 < total; k0 += CAP) {
-        const int k1 = min(total, k0 + CAP);
-        // Stage: quartic weights (pre-divided by h) and current factors per
-        // particle, from the records in cell order.
-        for (int j = k0 + t; j < k1; j += 256) {
-            int lo_c = 0, hi_c = nc;   // s_off[lo_c] <= j < s_off[hi_c]
-            while (hi_c - lo_c > 1) {
-                const int mid = (lo_c + hi_c) >> 1;
-                if (s_off[mid] <= j) lo_c = mid;
-                else hi_c = mid;
-            }
-            const DepRec<T> d = rec[bs.idx[s_first[lo_c] + (j - s_off[lo_c])]];
-            T w[5];
-            weights4<T>(d.xl, w);
+    Acc<T, KAHAN> acc[4];
+
+    auto pass = [&](auto a_const) {
+        constexpr int A = decltype(a_const)::value;
+        for (int i = t; i < nc; i += 256) {
+            Tap4<T> tp[5];
+            cell_taps<A, T>(rec, dense, s_first[i], s_cnt[i], s_dsl[i], inv_h, tp);
 #pragma unroll
-            for (int i = 0; i < 5; ++i) s_wx[i][j - k0] = w[i] * inv_h;
-            weights4<T>(d.yl, w);
-#pragma unroll
-            for (int i = 0; i < 5; ++i) s_wy[i][j - k0] = w[i] * inv_h;
-            s_c[0][j - k0] = d.c0;
-            s_c[1][j - k0] = d.c1;
-            s_c[2][j - k0] = d.c2;
-            s_c[3][j - k0] = d.c3;
-            s_cy[j - k0] = d.iy;
+            for (int b = 0; b < 5; ++b) s_tap[b][i] = tp[b];
         }
         __syncthreads();
-        // Accumulate source rows in ascending order: kx = ox - sx = 2 - kr.
+        const int r = ox - (A - 2) - hx0;
+        if (valid && r >= 0 && r < nrow) {
 #pragma unroll
-        for (int kr = 0; kr < 5; ++kr) {
-            const int b = max(lo[kr], k0), e = min(hi[kr], k1);
-            for (int j = b; j < e; ++j) {
-                const int jj = j - k0;
-                const T w = s_wx[4 - kr][jj] * s_wy[oy - s_cy[jj] + 2][jj];
-                a_ro.add(s_c[0][jj] * w);
-                a_jx.add(s_c[1][jj] * w);
-                a_jy.add(s_c[2][jj] * w);
-                a_jz.add(s_c[3][jj] * w);
+            for (int b = 0; b < 5; ++b) {
+                const int c = oy - (b - 2) - hy0;
+                if (c >= 0 && c < ncol) {
+                    const Tap4<T> v = s_tap[b][r * ncol + c];
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) acc[q].add(v.v[q]);
+                }
             }
         }
         __syncthreads();
-    }
-    // Dense source cells: their taps toward this cell.
-    if (valid) {
-        for (int kr = 0; kr < 5; ++kr) {
-            const int r = ox - 2 + kr - hx0;
-            if (r < 0 || r >= nrow) continue;
-            for (int c = cl; c < ch; ++c) {
-                const int ds = s_dsl[r * ncol + c];
-                if (ds < 0) continue;
-                const int tap = 5 * (ox - (hx0 + r) + 2) + (oy - (hy0 + c) + 2);
-                const T *v = dense + (size_t)ds * DENSE_VALUES + 4 * tap;
-                a_ro.add(v[0]);
-                a_jx.add(v[1]);
-                a_jy.add(v[2]);
-                a_jz.add(v[3]);
-            }
-        }
-    }
+    };
+    pass(std::integral_constant<int, 0>{});
+    pass(std::integral_constant<int, 1>{});
+    pass(std::integral_constant<int, 2>{});
+    pass(std::integral_constant<int, 3>{});
+    pass(std::integral_constant<int, 4>{});
 
     if (valid) {
         const int o = ox * N + oy;
-        ro[o] = (TG)a_ro.get();
-        jx[o] = (TG)a_jx.get();
-        jy[o] = (TG)a_jy.get();
-        jz[o] = (TG)a_jz.get();
+        ro[o] = (TG)acc[0].get();
+        jx[o] = (TG)acc[1].get();
+        jy[o] = (TG)acc[2].get();
+        jz[o] = (TG)acc[3].get();
         if (bg_ro != nullptr) {
             ro1[o] = bg_ro[o];
             jx1[o] = 0;
@@ -454,6 +458,7 @@ hipError_t launch_deposit(const PushGeom &g, const SortArrays<TP> &a, bool use_f
     const double *pz = use_full ? a.pz : a.pz_prev;
     const TP *q = use_full ? a.q : a.q_prev;
     const int pblocks = (int)((g.n + 255) / 256);
+    const int rec_words = (int)(sizeof(DepRec<TP>) / 16);
     if (g.n > 0) {
         if (!binned)
             k_bin<TP><<<pblocks, 256, 0, s>>>(g.n, g.h, N, a.x_init, a.y_init, xo, yo, px, py, pz,
@@ -463,6 +468,8 @@ hipError_t launch_deposit(const PushGeom &g, const SortArrays<TP> &a, bool use_f
         k_scan_final<<<nb, SCAN_THREADS, 0, s>>>(ncell, nb, bs.count, bs.blocks, bs.start);
         k_scatter<<<pblocks, 256, 0, s>>>(g.n, ncell, bs);
         k_order_cells<<<(ncell + 255) / 256, 256, 0, s>>>(ncell, bs);
+        k_sort_records<<<(int)std::min<int64_t>(8192, (g.n * rec_words + 255) / 256), 256, 0, s>>>(
+            ncell, bs, rec_words);
         k_order_big<<<256, BIG_SORT_THREADS, 0, s>>>(bs);
         k_dense_cells<TP><<<512, 256, 0, s>>>(g.h, bs);
     } else {

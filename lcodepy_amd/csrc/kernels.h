@@ -51,6 +51,8 @@ struct BinState {
     int bigcap;       // capacity of `big`
     int *dslot;       // N*N: a cell's slot in `big`, or -1
     void *dense;      // per dense cell: its 5 x 5 x 4 deposit (deposit.hip)
+    void *rec_sorted; // per list position of a non-dense cell: the record of the
+                      // particle idx names there (cell order)
 };
 
 // What the deposit needs of one binned plasma particle (weights.py:35-67):
@@ -61,6 +63,8 @@ template <typename T> struct alignas(16) DepRec {
     T xl, yl, c0, c1, c2, c3;
     int iy;
 };
+static_assert(sizeof(DepRec<float>) == 32 && sizeof(DepRec<double>) == 64,
+              "deposit records are copied as whole 16-byte words");
 
 // Capacity of the dense-cell queue: cells with more than 16 of the at most
 // `cap` particles.

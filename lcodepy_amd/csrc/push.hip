@@ -220,7 +220,9 @@ __global__ __launch_bounds__(256) void k_push(PushGeom g, Fields6<TF> f, SortArr
                                               BinState bs, const int *__restrict__ order)
 {
     // Threads past the end stay to the count so that whole waves reach it.
-    const int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    // Consecutive blocks of the order go to the same XCD, so each XCD's L2
+    // fetches the field rows of its own stretch of cells.
+    const int64_t j = xcd_swizzle(blockIdx.x, gridDim.x) * (int64_t)blockDim.x + threadIdx.x;
     int key = -1;
     if (j < g.n) {
         const int64_t k = order ? (int64_t)order[j] : j;   // the particle this thread pushes

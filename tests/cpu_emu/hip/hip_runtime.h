@@ -24,6 +24,7 @@
 struct dim3 { unsigned x = 1, y = 1, z = 1; dim3(unsigned a = 1, unsigned b = 1, unsigned c = 1) : x(a), y(b), z(c) {} };
 struct double2 { double x, y; };
 struct float2 { float x, y; };
+struct alignas(16) int4 { int x, y, z, w; };
 typedef int hipError_t;
 typedef void *hipStream_t;
 #define hipSuccess 0
