@@ -17,8 +17,9 @@
 // Kernels, all on plain global buffers and static LDS:
 //   k_rhs       R of all seven fields (one thread per grid point);
 //   k_gemm_mfma C_f = A_f B_f for the seven fields of a launch (64 x 64
-//               tiles through LDS, 16x16x4 matrix-core products), optionally
-//               scaled by M;
+//               tiles through LDS, 16x16x4 matrix-core products; the last
+//               row and column by the blocks of the last tile row and
+//               column), optionally scaled by M;
 //   k_assemble  outputs (full / half fields) and the Bz row sums;
 //   k_bz_finalize  mean of Bz removed, half-step Bz.
 #include "mfma.h"
@@ -50,31 +51,61 @@ __global__ __launch_bounds__(256) void k_rhs(RhsEval<TG, TPRED> rhs, TG *__restr
     work[f * nn + o] = in_rhs_domain(f, i, j, N) ? rhs(f, i, j) : TG(0);
 }
 
+// The solver-matrix factor of field f at spectral indices (kx, ky) = (gr, gc)
+// (initialization.py:15-67): zero where a DST index is an edge and at [0, 0]
+// for the Neumann field Bz.
+__device__ __forceinline__ double solver_factor(int f, int gr, int gc, int N, const ScaleGeom &sg)
+{
+    const int M = N - 1;
+    const bool dx = dst_along_x(f), dy = dst_along_y(f);
+    const double cc = (f == F_EZ || f == F_BZ || f == F_PHI) ? 0.0 : sg.c;
+    const bool ok = (!dx || (gr > 0 && gr < M)) && (!dy || (gc > 0 && gc < M)) &&
+                    !(f == F_BZ && gr == 0 && gc == 0);
+    return ok ? (1.0 / (sg.lam[gr] + sg.lam[gc] + cc)) / sg.d2 : 0.0;
+}
+
 constexpr int GBM = 64, GBN = 64, GBK = 32;
-constexpr int GLA = GBM * GBK / 256;   // A elements per thread per tile
-constexpr int GLB = GBK * GBN / 256;   // B elements per thread per tile
+constexpr int GRS = GBK + 4;            // LDS row stride of the A and B^T tiles
+constexpr int GLA = GBM * GBK / 256;    // A elements per thread per tile
+constexpr int GLB = GBK * GBN / 256;    // B elements per thread per tile
 
 // C_f = A_f B_f for the field f = blockIdx.z; all matrices N x N row-major.
-// 256 threads = 4 waves; the block's 64 x 64 tile is split 2 x 2 between the
-// waves, each wave computing 32 x 32 as 2 x 2 16x16x4 matrix-core products per
-// step of 4 along k.  Tiles of A (64 x 32) and B (32 x 64) pass through
-// static LDS; the next tiles' global loads are issued into registers before
-// the current tiles are multiplied.  Loads past the matrix edge read as zero
-// and stores past it are skipped.  With SCALE the result is multiplied by the
-// solver matrix of the field at (row, column) = (kx, ky)
-// (initialization.py:15-67).
+// Over k < M = N - 1 the products run on the matrix cores in 64 x 64 output
+// tiles covering [0, M) (rounded up to whole tiles); the k = M term is added
+// in the epilogue.  When M is a multiple of 64 the last row and column (index
+// M) lie just outside the tiles: the blocks of the last tile row also load
+// row M of A into LDS with each k tile and accumulate row M of C for their
+// 64 columns on the vector units from the same B tile (likewise the last tile
+// column for column M, and the corner block for C[M][M]).  Otherwise row and
+// column M lie inside the last tiles.  256 threads = 4 waves; a 64 x 64 tile
+// is split 2 x 2 between the waves, each computing 32 x 32 as 2 x 2 16x16x4
+// matrix-core products.  Tiles of A (64 x 32, row-major) and B (stored
+// transposed, 64 x 32) pass through double-buffered static LDS, one barrier
+// per tile; the next tiles' global loads are issued before the current tiles
+// are multiplied.  Within a tile, step s (0..7) of a wave's products takes
+// k = 8 lk + s for lane group lk = lane / 16, the same for A and B, so each
+// lane reads its eight A (and B) values of the tile as two 16-byte LDS
+// loads; the edge sums take the same k slice 8 (tid / 64) .. + 7 per thread
+// and add the four slices in a fixed order.  With SCALE the result is
+// multiplied by the solver matrix of the field (solver_factor).
 template <typename T, bool SCALE>
 __global__ __launch_bounds__(256) void k_gemm_mfma(GemmBatch<T> bt, int N, ScaleGeom sg)
 {
     using MM = Mfma16<T>;
     using acc_t = typename MM::acc_t;
-    __shared__ T As[GBM][GBK + 1];
-    __shared__ T Bs[GBK][GBN + 1];
+    __shared__ __attribute__((aligned(16))) T As[2][GBM][GRS];
+    __shared__ __attribute__((aligned(16))) T Bt[2][GBN][GRS];
+    __shared__ T Xa[2][GBK];        // row M of A, this k tile
+    __shared__ T Xb[2][GBK];        // column M of B, this k tile
+    __shared__ T red[2][4][GBM];    // edge partial sums per k slice
     const int f = blockIdx.z;
     const T *__restrict__ A = bt.A[f];
     const T *__restrict__ B = bt.B[f];
     T *__restrict__ C = bt.C[f];
+    const int M = N - 1;
     const int row0 = blockIdx.y * GBM, col0 = blockIdx.x * GBN;
+    const bool xrow = M % GBM == 0 && row0 + GBM == M;   // this block also sums row M
+    const bool xcol = M % GBN == 0 && col0 + GBN == M;   // ... and / or column M
     const int tid = threadIdx.x, lane = tid % 64, wave = tid / 64;
     const int wr = (wave / 2) * 32, wc = (wave % 2) * 32;   // the wave's 32 x 32 in the tile
     const int li = lane % 16, lk = lane / 16;
@@ -88,46 +119,74 @@ __global__ __launch_bounds__(256) void k_gemm_mfma(GemmBatch<T> bt, int N, Scale
         for (int tj = 0; tj < 2; ++tj)
 #pragma unroll
             for (int r = 0; r < 4; ++r) acc[ti][tj][r] = T(0);
+    T er = 0, ec = 0, ex = 0;   // row M / column M partial sums, corner
 
-    T ra[GLA], rb[GLB];
+    T ra[GLA], rb[GLB], xv = 0;
     auto load = [&](int k0) {
 #pragma unroll
         for (int q = 0; q < GLA; ++q) {
             const int gr = row0 + ar + 8 * q, gk = k0 + ak;
-            ra[q] = (gr < N && gk < N) ? A[(size_t)gr * N + gk] : T(0);
+            ra[q] = (gr < N && gk < M) ? A[(size_t)gr * N + gk] : T(0);
         }
 #pragma unroll
         for (int q = 0; q < GLB; ++q) {
             const int gk = k0 + bk + 4 * q, gc = col0 + bc;
-            rb[q] = (gk < N && gc < N) ? B[(size_t)gk * N + gc] : T(0);
+            rb[q] = (gk < M && gc < N) ? B[(size_t)gk * N + gc] : T(0);
+        }
+        if (xrow && tid < GBK) xv = k0 + tid < M ? A[(size_t)M * N + k0 + tid] : T(0);
+        if (xcol && tid >= 64 && tid < 64 + GBK) {
+            const int gk = k0 + tid - 64;
+            xv = gk < M ? B[(size_t)gk * N + M] : T(0);
         }
     };
+    auto store = [&](int buf) {
+#pragma unroll
+        for (int q = 0; q < GLA; ++q) As[buf][ar + 8 * q][ak] = ra[q];
+#pragma unroll
+        for (int q = 0; q < GLB; ++q) Bt[buf][bc][bk + 4 * q] = rb[q];
+        if (xrow && tid < GBK) Xa[buf][tid] = xv;
+        if (xcol && tid >= 64 && tid < 64 + GBK) Xb[buf][tid - 64] = xv;
+    };
     load(0);
-    for (int k0 = 0; k0 < N; k0 += GBK) {
+    store(0);
+    __syncthreads();
+    int buf = 0;
+    for (int k0 = 0; k0 < M; k0 += GBK) {
+        const bool more = k0 + GBK < M;
+        if (more) load(k0 + GBK);
+        T a[2][8], b[2][8];
 #pragma unroll
-        for (int q = 0; q < GLA; ++q) As[ar + 8 * q][ak] = ra[q];
+        for (int ti = 0; ti < 2; ++ti)
 #pragma unroll
-        for (int q = 0; q < GLB; ++q) Bs[bk + 4 * q][bc] = rb[q];
-        __syncthreads();
-        if (k0 + GBK < N) load(k0 + GBK);
+            for (int s = 0; s < 8; ++s) a[ti][s] = As[buf][wr + 16 * ti + li][8 * lk + s];
 #pragma unroll
-        for (int k4 = 0; k4 < GBK; k4 += 4) {
-            T a[2], b[2];
+        for (int tj = 0; tj < 2; ++tj)
 #pragma unroll
-            for (int ti = 0; ti < 2; ++ti) a[ti] = As[wr + 16 * ti + li][k4 + lk];
+            for (int s = 0; s < 8; ++s) b[tj][s] = Bt[buf][wc + 16 * tj + li][8 * lk + s];
 #pragma unroll
-            for (int tj = 0; tj < 2; ++tj) b[tj] = Bs[k4 + lk][wc + 16 * tj + li];
+        for (int s = 0; s < 8; ++s)
 #pragma unroll
             for (int ti = 0; ti < 2; ++ti)
 #pragma unroll
-                for (int tj = 0; tj < 2; ++tj) acc[ti][tj] = MM::mma(a[ti], b[tj], acc[ti][tj]);
+                for (int tj = 0; tj < 2; ++tj) acc[ti][tj] = MM::mma(a[ti][s], b[tj][s], acc[ti][tj]);
+        if (xrow) {
+#pragma unroll
+            for (int s = 0; s < 8; ++s) er += Xa[buf][8 * wave + s] * Bt[buf][lane][8 * wave + s];
+        }
+        if (xcol) {
+#pragma unroll
+            for (int s = 0; s < 8; ++s) ec += As[buf][lane][8 * wave + s] * Xb[buf][8 * wave + s];
+        }
+        if (xrow && xcol && tid == 0) {
+            for (int k = 0; k < GBK; ++k) ex += Xa[buf][k] * Xb[buf][k];
+        }
+        if (more) {
+            store(buf ^ 1);
+            buf ^= 1;
         }
         __syncthreads();
     }
 
-    const int M = N - 1;
-    const bool dx = dst_along_x(f), dy = dst_along_y(f);
-    const double cc = (f == F_EZ || f == F_BZ || f == F_PHI) ? 0.0 : sg.c;
 #pragma unroll
     for (int ti = 0; ti < 2; ++ti)
 #pragma unroll
@@ -137,17 +196,46 @@ __global__ __launch_bounds__(256) void k_gemm_mfma(GemmBatch<T> bt, int N, Scale
                 const int gr = row0 + wr + 16 * ti + MM::row(lk, r);
                 const int gc = col0 + wc + 16 * tj + li;
                 if (gr < N && gc < N) {
-                    T v = acc[ti][tj][r];
-                    if (SCALE) {
-                        // Spectral indices (kx, ky) = (gr, gc).
-                        const bool ok = (!dx || (gr > 0 && gr < M)) && (!dy || (gc > 0 && gc < M)) &&
-                                        !(f == F_BZ && gr == 0 && gc == 0);
-                        const double mul = ok ? (1.0 / (sg.lam[gr] + sg.lam[gc] + cc)) / sg.d2 : 0.0;
-                        v = (T)((double)v * mul);
-                    }
+                    // The k = M term.
+                    T v = acc[ti][tj][r] + A[(size_t)gr * N + M] * B[(size_t)M * N + gc];
+                    if (SCALE) v = (T)((double)v * solver_factor(f, gr, gc, N, sg));
                     C[(size_t)gr * N + gc] = v;
                 }
             }
+    if (xrow || xcol) {   // block-uniform
+        red[0][wave][lane] = er;
+        red[1][wave][lane] = ec;
+        __syncthreads();
+        if (tid < GBM) {
+            const T amm = A[(size_t)M * N + M];
+            if (xrow) {
+                const int gc = col0 + tid;
+                T v = ((red[0][0][tid] + red[0][1][tid]) + (red[0][2][tid] + red[0][3][tid])) +
+                      amm * B[(size_t)M * N + gc];
+                if (SCALE) v = (T)((double)v * solver_factor(f, M, gc, N, sg));
+                C[(size_t)M * N + gc] = v;
+            }
+            if (xcol) {
+                const int gr = row0 + tid;
+                T v = ((red[1][0][tid] + red[1][1][tid]) + (red[1][2][tid] + red[1][3][tid])) +
+                      A[(size_t)gr * N + M] * B[(size_t)M * N + M];
+                if (SCALE) v = (T)((double)v * solver_factor(f, gr, M, N, sg));
+                C[(size_t)gr * N + M] = v;
+            }
+            if (xrow && xcol && tid == 0) {
+                T v = ex + amm * B[(size_t)M * N + M];
+                if (SCALE) v = (T)((double)v * solver_factor(f, M, M, N, sg));
+                C[(size_t)M * N + M] = v;
+            }
+        }
+    }
+}
+
+template <typename T, bool SCALE>
+void gemm_launch(const GemmBatch<T> &bt, int N, const ScaleGeom &sg, hipStream_t s)
+{
+    const int tiles = (N - 1 + GBM - 1) / GBM;
+    k_gemm_mfma<T, SCALE><<<dim3(tiles, tiles, 7), 256, 0, s>>>(bt, N, sg);
 }
 
 // One wave per (row, field): outputs of the row and, for Bz, its row sum.
@@ -199,14 +287,13 @@ hipError_t launch_compute_fields_matrix(const MatrixGeom &g, const SolveIn<TG, T
         ix.A[f] = Tx;          ix.B[f] = w1 + f * nn; ix.C[f] = w2 + f * nn;
         iy.A[f] = w2 + f * nn; iy.B[f] = Ty_t; iy.C[f] = w1 + f * nn;
     }
-    const dim3 tiles((N + GBN - 1) / GBN, (N + GBM - 1) / GBM, 7);
     k_rhs<TG, TPRED><<<dim3((unsigned)((nn + 255) / 256), 7), 256, 0, s>>>(rhs, w1);
     hipError_t e;
     if (timer && (e = timer->begin(KC_SOLVE_GEMM, 4, s)) != hipSuccess) return e;
-    k_gemm_mfma<TG, false><<<tiles, 256, 0, s>>>(fy, N, sg);
-    k_gemm_mfma<TG, true><<<tiles, 256, 0, s>>>(sx, N, sg);
-    k_gemm_mfma<TG, false><<<tiles, 256, 0, s>>>(ix, N, sg);
-    k_gemm_mfma<TG, false><<<tiles, 256, 0, s>>>(iy, N, sg);
+    gemm_launch<TG, false>(fy, N, sg, s);
+    gemm_launch<TG, true>(sx, N, sg, s);
+    gemm_launch<TG, false>(ix, N, sg, s);
+    gemm_launch<TG, false>(iy, N, sg, s);
     if (timer && (e = timer->end(s)) != hipSuccess) return e;
     k_assemble<TG, TFULL, THALF><<<dim3((N + 3) / 4, 7), 256, 0, s>>>(w1, N, out, write_full,
                                                                        g.rowsum);
