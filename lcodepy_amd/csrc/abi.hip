@@ -21,6 +21,7 @@ struct lc3d_ctx {
     FftPlan plan{};
     // binning
     int *count = nullptr, *start = nullptr, *blocks = nullptr, *key = nullptr, *idx = nullptr;
+    int *rank = nullptr;
     unsigned *err = nullptr;
     void *rec = nullptr, *rec_sorted = nullptr;
     int *big = nullptr, *nbig = nullptr, *dslot = nullptr;
@@ -107,7 +108,7 @@ PushGeom push_geom(const lc3d_ctx *c, int64_t n)
 
 BinState bin_state(const lc3d_ctx *c)
 {
-    return BinState{c->count, c->start, c->blocks, c->key, c->idx, c->err, c->rec,
+    return BinState{c->count, c->start, c->blocks, c->key, c->rank, c->idx, c->err, c->rec,
                     std::max<int64_t>(1, c->cap), c->big, c->nbig, c->bigcap, c->dslot,
                     c->dense, c->rec_sorted};
 }
@@ -374,6 +375,7 @@ int lc3d_create(const lc3d_params *p, lc3d_ctx **out)
               hipMalloc(&c->start, (nn + 1) * sizeof(int)) == hipSuccess &&
               hipMalloc(&c->blocks, nblocks * sizeof(int)) == hipSuccess &&
               hipMalloc(&c->key, cap * sizeof(int)) == hipSuccess &&
+              hipMalloc(&c->rank, cap * sizeof(int)) == hipSuccess &&
               hipMalloc(&c->idx, cap * sizeof(int)) == hipSuccess &&
               hipMalloc(&c->err, sizeof(unsigned)) == hipSuccess &&
               hipMalloc(&c->rec, cap * deposit_record_bytes(f32)) == hipSuccess &&
@@ -467,6 +469,7 @@ void lc3d_destroy(lc3d_ctx *c)
     (void)hipFree(c->start);
     (void)hipFree(c->blocks);
     (void)hipFree(c->key);
+    (void)hipFree(c->rank);
     (void)hipFree(c->idx);
     (void)hipFree(c->err);
     (void)hipFree(c->rec);

@@ -98,6 +98,29 @@ __device__ __forceinline__ int xcd_swizzle(int b, int n) {
     return (b & 7) * (n8 >> 3) + (b >> 3);
 }
 
+// Counts the particles of a wave into their cells and returns each one's
+// rank in its cell (arrival order): consecutive lanes with the same key (the
+// common case when particles are visited in cell order) add their number
+// with one atomic, whose old value their run head shares with them.  Every
+// lane of the wave calls it; key < 0 is not counted and gets -1.
+__device__ __forceinline__ int count_rank(int *count, int key)
+{
+    const int lane = threadIdx.x & 63;
+    const int prev = __shfl_up(key, 1);
+    const bool head = key >= 0 && (lane == 0 || prev != key);
+    const unsigned long long ends = __ballot(head || key < 0);   // lanes that start a run
+    int old = 0;
+    if (head) {
+        const unsigned long long above = ends & ~((2ull << lane) - 1ull);
+        const int next = above ? __builtin_ctzll(above) : 64;
+        old = atomicAdd(&count[key], next - lane);
+    }
+    const unsigned long long upto = ends & ((2ull << lane) - 1ull);   // run starts <= lane
+    const int h = 63 - __builtin_clzll(upto);                        // this lane's run head
+    const int base = __shfl(old, h);
+    return key >= 0 ? base + (lane - h) : -1;
+}
+
 // Bins a plasma particle for the deposit: returns its cell key ix N + iy and
 // writes its record at rec[k], or returns -1 (with an error bit) when the 5x5
 // stencil would leave the grid or the position is not finite.  P Q Z M qv are

@@ -5,10 +5,12 @@
 // gather over cell-sorted particles:
 //
 //   1. bin      cell of every particle (fused into the push when possible),
-//               integer atomic count per cell;
-//   2. scan     exclusive prefix of the counts -> start of each cell's list;
-//   3. scatter  particle ids into their cell's list (the integer counters
-//               are counted back down to zero, ready for the next deposit);
+//               integer atomic count per cell (one atomic per run of equal
+//               cells in a wave), which also gives each particle its rank in
+//               its cell;
+//   2. scan     exclusive prefix of the counts -> start of each cell's list
+//               (the counts are cleared for the next deposit);
+//   3. scatter  particle ids into their cell's list at their ranks;
 //   4. order    each cell's list sorted by particle id, so the summation
 //               order below does not depend on atomic arrival order;
 //   5. taps     per dense cell (more than 16 particles) its 5 x 5 x 4 taps,
@@ -48,16 +50,19 @@ __global__ __launch_bounds__(256) void k_bin(int64_t n, double h, int N,
                                              const double *__restrict__ pz, const TP *__restrict__ q,
                                              const TP *__restrict__ m, BinState bs)
 {
+    // Threads past the end stay to the count so that whole waves reach it.
     const int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    if (k >= n) return;
     int key = -1;
-    if (q[k] != TP(0)) {
-        key = bin_particle<TP>(x_init[k] + xo[k], y_init[k] + yo[k], h, N, (double)px[k],
-                               (double)py[k], (double)pz[k], (double)m[k], (double)q[k],
-                               static_cast<DepRec<TP> *>(bs.rec), k, bs.err);
-        if (key >= 0) atomicAdd(&bs.count[key], 1);
+    if (k < n) {
+        if (q[k] != TP(0))
+            key = bin_particle<TP>(x_init[k] + xo[k], y_init[k] + yo[k], h, N, (double)px[k],
+                                   (double)py[k], (double)pz[k], (double)m[k], (double)q[k],
+                                   static_cast<DepRec<TP> *>(bs.rec), k, bs.err);
+        bs.key[k] = key;
     }
-    bs.key[k] = key;
+    // Unbinned particles are counted (and ranked) in count[N * N].
+    const int r = count_rank(bs.count, k < n ? (key >= 0 ? key : N * N) : -1);
+    if (k < n) bs.rank[k] = r;
 }
 
 // ---- exclusive scan of the N*N cell counts into start[0..N*N] --------------------
@@ -93,8 +98,10 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_scan_blocks(int ncell, const i
     if (threadIdx.x == 0) blocks[blockIdx.x] = total;
 }
 
+// Writes start[0..ncell] and clears the counts it read (ready for the next
+// binning pass).
 __global__ __launch_bounds__(SCAN_THREADS) void k_scan_final(int ncell, int nblocks,
-                                                             const int *__restrict__ count,
+                                                             int *__restrict__ count,
                                                              const int *__restrict__ blocks,
                                                              int *__restrict__ start)
 {
@@ -119,28 +126,24 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_scan_final(int ncell, int nblo
 #pragma unroll
     for (int i = 0; i < SCAN_PER; ++i) {
         if (base + i <= ncell) start[base + i] = run;
+        if (base + i < ncell) count[base + i] = 0;
         run += c[i];
     }
     (void)nblocks;
 }
 
-// ---- scatter ids into the cell lists, counting the counters back to zero --------
-// Places particle k in its cell's list; particles that are not binned (q = 0
-// or off the grid) go after all the lists, in arrival order, so that idx is a
-// permutation of the sort.  count[ncell] counts them and is cleared by
-// k_order_cells.
+// ---- scatter ids into the cell lists --------------------------------------------
+// Places particle k at its rank in its cell's list; particles that are not
+// binned (q = 0 or off the grid) go after all the lists, at their rank among
+// the unbinned ones, so that idx is a permutation of the sort.  count[ncell]
+// counts them and is cleared by k_order_cells.
 __global__ __launch_bounds__(256) void k_scatter(int64_t n, int ncell, BinState bs)
 {
     const int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (k >= n) return;
     if (k == 0) *bs.nbig = 0;   // k_order_cells refills the big-cell list
     const int key = bs.key[k];
-    if (key < 0) {
-        bs.idx[bs.start[ncell] + atomicAdd(&bs.count[ncell], 1)] = (int)k;
-        return;
-    }
-    const int slot = bs.start[key] + atomicSub(&bs.count[key], 1) - 1;
-    bs.idx[slot] = (int)k;
+    bs.idx[bs.start[key < 0 ? ncell : key] + bs.rank[k]] = (int)k;
 }
 
 // ---- sort each cell's list by particle id ---------------------------------------

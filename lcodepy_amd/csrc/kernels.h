@@ -38,10 +38,14 @@ template <typename TP> struct SortArrays {
 
 // Binning state shared by push and deposit (owned by the context).
 struct BinState {
-    int *count;       // N*N + 1, zero between deposits
+    int *count;       // N*N + 1, zero between deposits (the scan and the list
+                      // sort clear them)
     int *start;       // N*N + 1
     int *blocks;      // scan block sums
     int *key;         // per particle: cell index or -1
+    int *rank;        // per particle: its place among the particles counted into
+                      // the same cell (or among the unbinned ones, counted in
+                      // count[N*N]), in arrival order
     int *idx;         // per particle: particle ids sorted by cell, then the unbinned ones
     unsigned *err;    // sticky error bits
     void *rec;        // per particle: deposit records (deposit.hip)

@@ -198,23 +198,6 @@ __device__ __forceinline__ int push_one(const PushGeom &g, const Fields6<TF> &f,
     return key;
 }
 
-// Adds the particles of the wave to their cells' counts: consecutive lanes
-// with the same cell (the common case when particles are pushed in cell
-// order) add their number with one atomic.  Every lane of the wave calls it;
-// key < 0 adds nothing.
-__device__ __forceinline__ void count_keys(int *count, int key)
-{
-    const int lane = threadIdx.x & 63;
-    const int prev = __shfl_up(key, 1);
-    const bool head = key >= 0 && (lane == 0 || prev != key);
-    const unsigned long long ends = __ballot(head || key < 0);   // lanes that start a new run
-    if (head) {
-        const unsigned long long above = ends & ~((2ull << lane) - 1ull);
-        const int next = above ? __builtin_ctzll(above) : 64;
-        atomicAdd(&count[key], next - lane);
-    }
-}
-
 template <typename TP, typename TF, bool FIRST, bool BIN>
 __global__ __launch_bounds__(256) void k_push(PushGeom g, Fields6<TF> f, SortArrays<TP> a,
                                               BinState bs, const int *__restrict__ order)
@@ -224,11 +207,17 @@ __global__ __launch_bounds__(256) void k_push(PushGeom g, Fields6<TF> f, SortArr
     // fetches the field rows of its own stretch of cells.
     const int64_t j = xcd_swizzle(blockIdx.x, gridDim.x) * (int64_t)blockDim.x + threadIdx.x;
     int key = -1;
+    int64_t k = 0;
     if (j < g.n) {
-        const int64_t k = order ? (int64_t)order[j] : j;   // the particle this thread pushes
+        k = order ? (int64_t)order[j] : j;   // the particle this thread pushes
         key = push_one<TP, TF, FIRST, BIN>(g, f, a, bs, k);
     }
-    if (BIN) count_keys(bs.count, key);
+    if (BIN) {
+        // Unbinned particles are counted (and ranked) in count[N * N].
+        const int ck = j < g.n ? (key >= 0 ? key : g.N * g.N) : -1;
+        const int r = count_rank(bs.count, ck);
+        if (j < g.n) bs.rank[k] = r;
+    }
 }
 
 

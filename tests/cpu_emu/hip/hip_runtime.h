@@ -64,6 +64,14 @@ inline int __shfl_up(int v, unsigned d) {
     emu_sync_wave();
     return r;
 }
+inline int __shfl(int v, int src) {
+    int t = threadIdx.x, w = t / 64;
+    emu_shfl_i[t] = v;
+    emu_sync_wave();
+    int r = emu_shfl_i[(src & 63) + w * 64];
+    emu_sync_wave();
+    return r;
+}
 inline unsigned long long __ballot(int pred) {
     int t = threadIdx.x, w = t / 64;
     emu_shfl_i[t] = pred != 0;
