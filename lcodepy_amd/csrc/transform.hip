@@ -64,10 +64,17 @@ __device__ __forceinline__ double solver_factor(int f, int gr, int gc, int N, co
     return ok ? (1.0 / (sg.lam[gr] + sg.lam[gc] + cc)) / sg.d2 : 0.0;
 }
 
-constexpr int GBM = 64, GBN = 64, GBK = 32;
-constexpr int GRS = GBK + 4;            // LDS row stride of the A and B^T tiles
-constexpr int GLA = GBM * GBK / 256;    // A elements per thread per tile
-constexpr int GLB = GBK * GBN / 256;    // B elements per thread per tile
+constexpr int GBM = 64, GBN = 64;
+// Depth of a k tile: 64 in float32, 32 in float64 (two blocks per CU fit
+// the double-buffered LDS tiles either way).
+template <typename T> struct GemmTile {
+    static constexpr int BK = sizeof(T) == 4 ? 64 : 32;
+    static constexpr int RS = BK + 4;            // LDS row stride of the A and B^T tiles
+    static constexpr int NA = GBM * BK / 256;    // A elements per thread per tile
+    static constexpr int NB = BK * GBN / 256;    // B elements per thread per tile
+    static constexpr int RA = 256 / BK;          // A rows loaded per pass of the block
+    static constexpr int KS = BK / 4;            // products per wave and 16 x 16 tile
+};
 
 // C_f = A_f B_f for the field f = blockIdx.z; all matrices N x N row-major.
 // Over k < M = N - 1 the products run on the matrix cores in 64 x 64 output
@@ -79,20 +86,23 @@ constexpr int GLB = GBK * GBN / 256;    // B elements per thread per tile
 // column for column M, and the corner block for C[M][M]).  Otherwise row and
 // column M lie inside the last tiles.  256 threads = 4 waves; a 64 x 64 tile
 // is split 2 x 2 between the waves, each computing 32 x 32 as 2 x 2 16x16x4
-// matrix-core products.  Tiles of A (64 x 32, row-major) and B (stored
-// transposed, 64 x 32) pass through double-buffered static LDS, one barrier
-// per tile; the next tiles' global loads are issued before the current tiles
-// are multiplied.  Within a tile, step s (0..7) of a wave's products takes
-// k = 8 lk + s for lane group lk = lane / 16, the same for A and B, so each
-// lane reads its eight A (and B) values of the tile as two 16-byte LDS
-// loads; the edge sums take the same k slice 8 (tid / 64) .. + 7 per thread
-// and add the four slices in a fixed order.  With SCALE the result is
+// matrix-core products.  Tiles of A (64 x BK, row-major) and B (stored
+// transposed, 64 x BK) pass through double-buffered static LDS, one barrier
+// per tile (depth BK, GemmTile); the next tiles' global loads are issued
+// before the current tiles are multiplied.  Within a tile, step s (0..KS-1,
+// KS = BK / 4) of a wave's products takes k = KS lk + s for lane group
+// lk = lane / 16, the same for A and B, so each lane reads its KS A (and B)
+// values of the tile as 16-byte LDS loads; the edge sums take the k slice
+// KS (tid / 64) .. + KS - 1 per thread and add the four slices in a fixed
+// order.  With SCALE the result is
 // multiplied by the solver matrix of the field (solver_factor).
 template <typename T, bool SCALE>
 __global__ __launch_bounds__(256) void k_gemm_mfma(GemmBatch<T> bt, int N, ScaleGeom sg)
 {
     using MM = Mfma16<T>;
     using acc_t = typename MM::acc_t;
+    constexpr int GBK = GemmTile<T>::BK, GRS = GemmTile<T>::RS, GLA = GemmTile<T>::NA,
+                  GLB = GemmTile<T>::NB, GRA = GemmTile<T>::RA, KS = GemmTile<T>::KS;
     __shared__ __attribute__((aligned(16))) T As[2][GBM][GRS];
     __shared__ __attribute__((aligned(16))) T Bt[2][GBN][GRS];
     __shared__ T Xa[2][GBK];        // row M of A, this k tile
@@ -109,7 +119,7 @@ __global__ __launch_bounds__(256) void k_gemm_mfma(GemmBatch<T> bt, int N, Scale
     const int tid = threadIdx.x, lane = tid % 64, wave = tid / 64;
     const int wr = (wave / 2) * 32, wc = (wave % 2) * 32;   // the wave's 32 x 32 in the tile
     const int li = lane % 16, lk = lane / 16;
-    // Loader coordinates: A rows ar + 8 q, column ak; B rows bk + 4 q, column bc.
+    // Loader coordinates: A rows ar + GRA q, column ak; B rows bk + 4 q, column bc.
     const int ar = tid / GBK, ak = tid % GBK;
     const int bk = tid / GBN, bc = tid % GBN;
     acc_t acc[2][2];
@@ -125,7 +135,7 @@ __global__ __launch_bounds__(256) void k_gemm_mfma(GemmBatch<T> bt, int N, Scale
     auto load = [&](int k0) {
 #pragma unroll
         for (int q = 0; q < GLA; ++q) {
-            const int gr = row0 + ar + 8 * q, gk = k0 + ak;
+            const int gr = row0 + ar + GRA * q, gk = k0 + ak;
             ra[q] = (gr < N && gk < M) ? A[(size_t)gr * N + gk] : T(0);
         }
 #pragma unroll
@@ -141,7 +151,7 @@ __global__ __launch_bounds__(256) void k_gemm_mfma(GemmBatch<T> bt, int N, Scale
     };
     auto store = [&](int buf) {
 #pragma unroll
-        for (int q = 0; q < GLA; ++q) As[buf][ar + 8 * q][ak] = ra[q];
+        for (int q = 0; q < GLA; ++q) As[buf][ar + GRA * q][ak] = ra[q];
 #pragma unroll
         for (int q = 0; q < GLB; ++q) Bt[buf][bc][bk + 4 * q] = rb[q];
         if (xrow && tid < GBK) Xa[buf][tid] = xv;
@@ -154,28 +164,28 @@ __global__ __launch_bounds__(256) void k_gemm_mfma(GemmBatch<T> bt, int N, Scale
     for (int k0 = 0; k0 < M; k0 += GBK) {
         const bool more = k0 + GBK < M;
         if (more) load(k0 + GBK);
-        T a[2][8], b[2][8];
+        T a[2][KS], b[2][KS];
 #pragma unroll
         for (int ti = 0; ti < 2; ++ti)
 #pragma unroll
-            for (int s = 0; s < 8; ++s) a[ti][s] = As[buf][wr + 16 * ti + li][8 * lk + s];
+            for (int s = 0; s < KS; ++s) a[ti][s] = As[buf][wr + 16 * ti + li][KS * lk + s];
 #pragma unroll
         for (int tj = 0; tj < 2; ++tj)
 #pragma unroll
-            for (int s = 0; s < 8; ++s) b[tj][s] = Bt[buf][wc + 16 * tj + li][8 * lk + s];
+            for (int s = 0; s < KS; ++s) b[tj][s] = Bt[buf][wc + 16 * tj + li][KS * lk + s];
 #pragma unroll
-        for (int s = 0; s < 8; ++s)
+        for (int s = 0; s < KS; ++s)
 #pragma unroll
             for (int ti = 0; ti < 2; ++ti)
 #pragma unroll
                 for (int tj = 0; tj < 2; ++tj) acc[ti][tj] = MM::mma(a[ti][s], b[tj][s], acc[ti][tj]);
         if (xrow) {
 #pragma unroll
-            for (int s = 0; s < 8; ++s) er += Xa[buf][8 * wave + s] * Bt[buf][lane][8 * wave + s];
+            for (int s = 0; s < KS; ++s) er += Xa[buf][KS * wave + s] * Bt[buf][lane][KS * wave + s];
         }
         if (xcol) {
 #pragma unroll
-            for (int s = 0; s < 8; ++s) ec += As[buf][lane][8 * wave + s] * Xb[buf][8 * wave + s];
+            for (int s = 0; s < KS; ++s) ec += As[buf][lane][KS * wave + s] * Xb[buf][KS * wave + s];
         }
         if (xrow && xcol && tid == 0) {
             for (int k = 0; k < GBK; ++k) ex += Xa[buf][k] * Xb[buf][k];
@@ -235,6 +245,7 @@ template <typename T, bool SCALE>
 void gemm_launch(const GemmBatch<T> &bt, int N, const ScaleGeom &sg, hipStream_t s)
 {
     const int tiles = (N - 1 + GBM - 1) / GBM;
+    static_assert(GemmTile<T>::BK <= 64, "the edge loaders use threads 0..127");
     k_gemm_mfma<T, SCALE><<<dim3(tiles, tiles, 7), 256, 0, s>>>(bt, N, sg);
 }
 
