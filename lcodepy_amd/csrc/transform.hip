@@ -23,6 +23,7 @@
 //   k_assemble  outputs (full / half fields) and the Bz row sums;
 //   k_bz_finalize  mean of Bz removed, half-step Bz.
 #include "mfma.h"
+#include <type_traits>
 #include "solve_common.h"
 
 namespace lc3d {
@@ -88,8 +89,8 @@ template <typename T> struct GemmTile {
 // is split 2 x 2 between the waves, each computing 32 x 32 as 2 x 2 16x16x4
 // matrix-core products.  Tiles of A (64 x BK, row-major) and B (stored
 // transposed, 64 x BK) pass through double-buffered static LDS, one barrier
-// per tile (depth BK, GemmTile); the next tiles' global loads are issued
-// before the current tiles are multiplied.  Within a tile, step s (0..KS-1,
+// per tile (depth BK, GemmTile); a tile's global loads are issued two tiles
+// before it is multiplied (two register sets).  Within a tile, step s (0..KS-1,
 // KS = BK / 4) of a wave's products takes k = KS lk + s for lane group
 // lk = lane / 16, the same for A and B, so each lane reads its KS A (and B)
 // values of the tile as 16-byte LDS loads; the edge sums take the k slice
@@ -97,7 +98,7 @@ template <typename T> struct GemmTile {
 // order.  With SCALE the result is
 // multiplied by the solver matrix of the field (solver_factor).
 template <typename T, bool SCALE>
-__global__ __launch_bounds__(256) void k_gemm_mfma(GemmBatch<T> bt, int N, ScaleGeom sg)
+__global__ __launch_bounds__(256, 2) void k_gemm_mfma(GemmBatch<T> bt, int N, ScaleGeom sg)
 {
     using MM = Mfma16<T>;
     using acc_t = typename MM::acc_t;
@@ -131,70 +132,86 @@ __global__ __launch_bounds__(256) void k_gemm_mfma(GemmBatch<T> bt, int N, Scale
             for (int r = 0; r < 4; ++r) acc[ti][tj][r] = T(0);
     T er = 0, ec = 0, ex = 0;   // row M / column M partial sums, corner
 
-    T ra[GLA], rb[GLB], xv = 0;
-    auto load = [&](int k0) {
+    // Two register sets: tile u is loaded into set u % 2, two tiles ahead of
+    // the tile being multiplied.
+    T ra[2][GLA], rb[2][GLB], xv[2] = {T(0), T(0)};
+    auto load = [&](auto set, int k0) {
+        constexpr int S = decltype(set)::value;
 #pragma unroll
         for (int q = 0; q < GLA; ++q) {
             const int gr = row0 + ar + GRA * q, gk = k0 + ak;
-            ra[q] = (gr < N && gk < M) ? A[(size_t)gr * N + gk] : T(0);
+            ra[S][q] = (gr < N && gk < M) ? A[(size_t)gr * N + gk] : T(0);
         }
 #pragma unroll
         for (int q = 0; q < GLB; ++q) {
             const int gk = k0 + bk + 4 * q, gc = col0 + bc;
-            rb[q] = (gk < M && gc < N) ? B[(size_t)gk * N + gc] : T(0);
+            rb[S][q] = (gk < M && gc < N) ? B[(size_t)gk * N + gc] : T(0);
         }
-        if (xrow && tid < GBK) xv = k0 + tid < M ? A[(size_t)M * N + k0 + tid] : T(0);
+        if (xrow && tid < GBK) xv[S] = k0 + tid < M ? A[(size_t)M * N + k0 + tid] : T(0);
         if (xcol && tid >= 64 && tid < 64 + GBK) {
             const int gk = k0 + tid - 64;
-            xv = gk < M ? B[(size_t)gk * N + M] : T(0);
+            xv[S] = gk < M ? B[(size_t)gk * N + M] : T(0);
         }
     };
-    auto store = [&](int buf) {
+    auto store = [&](auto set) {   // set S into LDS buffer S
+        constexpr int S = decltype(set)::value;
 #pragma unroll
-        for (int q = 0; q < GLA; ++q) As[buf][ar + GRA * q][ak] = ra[q];
+        for (int q = 0; q < GLA; ++q) As[S][ar + GRA * q][ak] = ra[S][q];
 #pragma unroll
-        for (int q = 0; q < GLB; ++q) Bt[buf][bc][bk + 4 * q] = rb[q];
-        if (xrow && tid < GBK) Xa[buf][tid] = xv;
-        if (xcol && tid >= 64 && tid < 64 + GBK) Xb[buf][tid - 64] = xv;
+        for (int q = 0; q < GLB; ++q) Bt[S][bc][bk + 4 * q] = rb[S][q];
+        if (xrow && tid < GBK) Xa[S][tid] = xv[S];
+        if (xcol && tid >= 64 && tid < 64 + GBK) Xb[S][tid - 64] = xv[S];
     };
-    load(0);
-    store(0);
-    __syncthreads();
-    int buf = 0;
-    for (int k0 = 0; k0 < M; k0 += GBK) {
-        const bool more = k0 + GBK < M;
-        if (more) load(k0 + GBK);
-        T a[2][KS], b[2][KS];
+    // Tile t (of nt): multiply LDS buffer t % 2 while tile t + 2 loads into
+    // register set t % 2, then move tile t + 1 from set (t + 1) % 2 into its
+    // LDS buffer.
+    auto step = [&](auto par, int t, int nt) {
+        constexpr int P = decltype(par)::value;
+        using Other = std::integral_constant<int, P ^ 1>;
+        if (t + 2 < nt) load(par, (t + 2) * GBK);
 #pragma unroll
-        for (int ti = 0; ti < 2; ++ti)
-#pragma unroll
-            for (int s = 0; s < KS; ++s) a[ti][s] = As[buf][wr + 16 * ti + li][KS * lk + s];
-#pragma unroll
-        for (int tj = 0; tj < 2; ++tj)
-#pragma unroll
-            for (int s = 0; s < KS; ++s) b[tj][s] = Bt[buf][wc + 16 * tj + li][KS * lk + s];
-#pragma unroll
-        for (int s = 0; s < KS; ++s)
+        for (int s4 = 0; s4 < KS; s4 += 4) {   // four steps per 16-byte LDS read
+            T a[2][4], b[2][4];
 #pragma unroll
             for (int ti = 0; ti < 2; ++ti)
 #pragma unroll
-                for (int tj = 0; tj < 2; ++tj) acc[ti][tj] = MM::mma(a[ti][s], b[tj][s], acc[ti][tj]);
+                for (int s = 0; s < 4; ++s) a[ti][s] = As[P][wr + 16 * ti + li][KS * lk + s4 + s];
+#pragma unroll
+            for (int tj = 0; tj < 2; ++tj)
+#pragma unroll
+                for (int s = 0; s < 4; ++s) b[tj][s] = Bt[P][wc + 16 * tj + li][KS * lk + s4 + s];
+#pragma unroll
+            for (int s = 0; s < 4; ++s)
+#pragma unroll
+                for (int ti = 0; ti < 2; ++ti)
+#pragma unroll
+                    for (int tj = 0; tj < 2; ++tj)
+                        acc[ti][tj] = MM::mma(a[ti][s], b[tj][s], acc[ti][tj]);
+        }
         if (xrow) {
 #pragma unroll
-            for (int s = 0; s < KS; ++s) er += Xa[buf][KS * wave + s] * Bt[buf][lane][KS * wave + s];
+            for (int s = 0; s < KS; ++s) er += Xa[P][KS * wave + s] * Bt[P][lane][KS * wave + s];
         }
         if (xcol) {
 #pragma unroll
-            for (int s = 0; s < KS; ++s) ec += As[buf][lane][KS * wave + s] * Xb[buf][KS * wave + s];
+            for (int s = 0; s < KS; ++s) ec += As[P][lane][KS * wave + s] * Xb[P][KS * wave + s];
         }
         if (xrow && xcol && tid == 0) {
-            for (int k = 0; k < GBK; ++k) ex += Xa[buf][k] * Xb[buf][k];
+            for (int k = 0; k < GBK; ++k) ex += Xa[P][k] * Xb[P][k];
         }
-        if (more) {
-            store(buf ^ 1);
-            buf ^= 1;
-        }
+        if (t + 1 < nt) store(Other{});
         __syncthreads();
+    };
+    using Zero = std::integral_constant<int, 0>;
+    using One = std::integral_constant<int, 1>;
+    const int nt = (M + GBK - 1) / GBK;   // k tiles (M >= 1)
+    load(Zero{}, 0);
+    store(Zero{});
+    if (nt > 1) load(One{}, GBK);
+    __syncthreads();
+    for (int t = 0; t < nt; t += 2) {
+        step(Zero{}, t, nt);
+        if (t + 1 < nt) step(One{}, t + 1, nt);
     }
 
 #pragma unroll

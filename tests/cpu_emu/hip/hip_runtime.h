@@ -35,7 +35,7 @@ inline const char *hipGetErrorString(hipError_t) { return "emu"; }
 #define __device__
 #define __host__
 #define __forceinline__ inline
-#define __launch_bounds__(x)
+#define __launch_bounds__(...)
 #define __shared__ static thread_local
 #define __align__(n) alignas(n)
 #define __restrict__
