@@ -315,11 +315,39 @@ template <typename T> struct alignas(4 * sizeof(T)) Tap4 {
 // particles, in list order, of c_q (w_A(xl) / h) (w_b(yl) / h), from the
 // records in list order (rec = rec_sorted, first = the cell's list start),
 // or the precomputed block of a dense cell.
+// Records of a halo cell's first particles kept in registers for all five
+// passes of the gather.
+template <typename T> struct TapPrefetch {
+    static constexpr int PF = sizeof(T) == 4 ? 4 : 2;
+};
+
+template <int A, typename T>
+__device__ __forceinline__ void add_taps(const DepRec<T> &d, T inv_h, Tap4<T> tp[5])
+{
+    const T wx = w4_tap<A>(d.xl) * inv_h;
+    T wy[5];
+    weights4<T>(d.yl, wy);
+#pragma unroll
+    for (int b = 0; b < 5; ++b) {
+        const T w = wx * (wy[b] * inv_h);
+        tp[b].v[0] += d.c0 * w;
+        tp[b].v[1] += d.c1 * w;
+        tp[b].v[2] += d.c2 * w;
+        tp[b].v[3] += d.c3 * w;
+    }
+}
+
+// The five taps toward x offset A - 2 (target cells (ix + A - 2, iy + b - 2),
+// b = 0..4) of one source cell, for the four currents: the sum over its
+// particles, in list order, of c_q (w_A(xl) / h) (w_b(yl) / h) -- the first
+// PF records from the registers `pre`, the others from rec (= rec_sorted,
+// first = the cell's list start) -- or the precomputed block of a dense cell.
 template <int A, typename T>
 __device__ __forceinline__ void cell_taps(const DepRec<T> *__restrict__ rec,
                                           const T *__restrict__ dense, int first, int n, int dsl,
-                                          T inv_h, Tap4<T> tp[5])
+                                          const DepRec<T> *pre, T inv_h, Tap4<T> tp[5])
 {
+    constexpr int PF = TapPrefetch<T>::PF;
 #pragma unroll
     for (int b = 0; b < 5; ++b)
 #pragma unroll
@@ -332,8 +360,11 @@ __device__ __forceinline__ void cell_taps(const DepRec<T> *__restrict__ rec,
             for (int q = 0; q < 4; ++q) tp[b].v[q] = v[4 * b + q];
         return;
     }
-    // Up to four records in flight at a time.
-    for (int j0 = 0; j0 < n; j0 += 4) {
+#pragma unroll
+    for (int k = 0; k < PF; ++k)
+        if (k < n) add_taps<A, T>(pre[k], inv_h, tp);
+    // The rest, up to four records in flight at a time.
+    for (int j0 = PF; j0 < n; j0 += 4) {
         DepRec<T> d[4];
 #pragma unroll
         for (int k = 0; k < 4; ++k)
@@ -341,17 +372,7 @@ __device__ __forceinline__ void cell_taps(const DepRec<T> *__restrict__ rec,
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
             if (j0 + k >= n) break;
-            const T wx = w4_tap<A>(d[k].xl) * inv_h;
-            T wy[5];
-            weights4<T>(d[k].yl, wy);
-#pragma unroll
-            for (int b = 0; b < 5; ++b) {
-                const T w = wx * (wy[b] * inv_h);
-                tp[b].v[0] += d[k].c0 * w;
-                tp[b].v[1] += d[k].c1 * w;
-                tp[b].v[2] += d[k].c2 * w;
-                tp[b].v[3] += d[k].c3 * w;
-            }
+            add_taps<A, T>(d[k], inv_h, tp);
         }
     }
 }
@@ -402,14 +423,33 @@ __global__ __launch_bounds__(256) void k_deposit_gather(
     const bool valid = ox < N && oy < N;
     const T inv_h = T(1.0 / h);
     Acc<T, KAHAN> acc[4];
+    static_assert(DEP_HALO * DEP_HALO <= 512, "two halo cells per thread");
+
+    // This thread's halo cells (t and t + 256; nc <= 400) and their first
+    // records, loaded once for the five passes.
+    constexpr int PF = TapPrefetch<T>::PF;
+    DepRec<T> pre[2][PF];
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+        const int i = t + 256 * u;
+        if (i < nc && s_dsl[i] < 0) {
+#pragma unroll
+            for (int k = 0; k < PF; ++k)
+                if (k < s_cnt[i]) pre[u][k] = rec[s_first[i] + k];
+        }
+    }
 
     auto pass = [&](auto a_const) {
         constexpr int A = decltype(a_const)::value;
-        for (int i = t; i < nc; i += 256) {
-            Tap4<T> tp[5];
-            cell_taps<A, T>(rec, dense, s_first[i], s_cnt[i], s_dsl[i], inv_h, tp);
 #pragma unroll
-            for (int b = 0; b < 5; ++b) s_tap[b][i] = tp[b];
+        for (int u = 0; u < 2; ++u) {
+            const int i = t + 256 * u;
+            if (i < nc) {
+                Tap4<T> tp[5];
+                cell_taps<A, T>(rec, dense, s_first[i], s_cnt[i], s_dsl[i], pre[u], inv_h, tp);
+#pragma unroll
+                for (int b = 0; b < 5; ++b) s_tap[b][i] = tp[b];
+            }
         }
         __syncthreads();
         const int r = ox - (A - 2) - hx0;
