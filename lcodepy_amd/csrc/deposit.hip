@@ -11,10 +11,13 @@
 //   2. scan     exclusive prefix of the counts -> start of each cell's list
 //               (the counts are cleared for the next deposit);
 //   3. scatter  particle ids into their cell's list at their ranks;
-//   4. order    each cell's list sorted by particle id, so the summation
-//               order below does not depend on atomic arrival order;
-//   5. taps     per dense cell (more than 16 particles) its 5 x 5 x 4 taps,
-//               by one workgroup with a fixed-order reduction;
+//   4. order    each cell's list sorted by particle id (by the cell's thread,
+//               or for a dense cell by a workgroup in step 5), so the
+//               summation order below does not depend on atomic arrival
+//               order;
+//   5. taps     per dense cell (more than 16 particles) its list sorted and
+//               its 5 x 5 x 4 taps, by one workgroup with a fixed-order
+//               reduction;
 //   6. gather   one workgroup per 16x16 tile of output cells forms, in five
 //               passes over the x offset, the taps of the 20x20 source
 //               cells around it in LDS; each thread adds, for its own output
@@ -149,7 +152,7 @@ __global__ __launch_bounds__(256) void k_scatter(int64_t n, int ncell, BinState 
 // ---- sort each cell's list by particle id ---------------------------------------
 // Lists of up to SMALL_CELL particles are sorted by their cell's thread
 // (insertion sort).  Longer ones make the cell dense: it is queued (its slot
-// recorded in dslot) for k_order_big and k_dense_cells.
+// recorded in dslot) for k_dense_cells, which sorts it.
 __global__ __launch_bounds__(256) void k_order_cells(int ncell, BinState bs)
 {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
@@ -195,52 +198,21 @@ __global__ __launch_bounds__(256) void k_sort_records(int ncell, BinState bs, in
     }
 }
 
-// Bitonic sort, in LDS, of the queued long lists: one workgroup per list at
-// a time, the grid striding over the queue.
-constexpr int BIG_SORT_THREADS = 256;
-__global__ __launch_bounds__(BIG_SORT_THREADS) void k_order_big(BinState bs)
+// ---- dense cells: sort and taps ------------------------------------------------------
+// One workgroup per dense cell, the grid striding over the queue.  The cell's
+// list is sorted by particle id with a bitonic sort in LDS (lists of up to
+// MAX_SORTED_CELL; longer ones keep arrival order) and written back, so the
+// next push visits the cell's particles in id order.  Then thread t sums its
+// contiguous segment of the sorted list into the 5 x 5 taps x 4 currents
+// around the cell; the 256 partial sums are combined by a butterfly within
+// each wave and then the four waves in a fixed order.  out[slot][tap][q],
+// tap = 5 a + b for the target cell (ix + a - 2, iy + b - 2), q = ro jx jy jz.
+constexpr int DENSE_VALUES = 25 * 4;
+constexpr int DENSE_THREADS = 256;
+template <typename T>
+__global__ __launch_bounds__(DENSE_THREADS) void k_dense_cells(double h, BinState bs)
 {
     __shared__ int s[MAX_SORTED_CELL];
-    const int nbig = min(*bs.nbig, bs.bigcap);
-    const int t = threadIdx.x;
-    for (int b = blockIdx.x; b < nbig; b += gridDim.x) {
-        const int c = bs.big[b];
-        const int st = bs.start[c], n = bs.start[c + 1] - st;   // n > SMALL_CELL
-        if (n > MAX_SORTED_CELL) continue;                       // summed in arrival order
-        int P = 1;
-        while (P < n) P <<= 1;
-        for (int i = t; i < P; i += BIG_SORT_THREADS) s[i] = i < n ? bs.idx[st + i] : INT_MAX;
-        __syncthreads();
-        for (int k = 2; k <= P; k <<= 1) {
-            for (int j = k >> 1; j > 0; j >>= 1) {
-                for (int i = t; i < P; i += BIG_SORT_THREADS) {
-                    const int l = i ^ j;
-                    if (l > i) {
-                        const int x = s[i], y = s[l];
-                        if ((x > y) == ((i & k) == 0)) {
-                            s[i] = y;
-                            s[l] = x;
-                        }
-                    }
-                }
-                __syncthreads();
-            }
-        }
-        for (int i = t; i < n; i += BIG_SORT_THREADS) bs.idx[st + i] = s[i];
-        __syncthreads();
-    }
-}
-
-// ---- deposit of the dense cells ---------------------------------------------------
-// One workgroup per dense cell: thread t sums its contiguous segment of the
-// cell's (id-ordered) particles into the 5 x 5 taps x 4 currents around the
-// cell; the 256 partial sums are combined by a butterfly within each wave and
-// then the four waves in a fixed order.  out[slot][tap][q], tap = 5 a + b for
-// the target cell (ix + a - 2, iy + b - 2), q = ro jx jy jz.
-constexpr int DENSE_VALUES = 25 * 4;
-template <typename T>
-__global__ __launch_bounds__(256) void k_dense_cells(double h, BinState bs)
-{
     __shared__ T red[4][DENSE_VALUES];
     const int nbig = min(*bs.nbig, bs.bigcap);
     const DepRec<T> *__restrict__ rec = static_cast<const DepRec<T> *>(bs.rec);
@@ -249,14 +221,37 @@ __global__ __launch_bounds__(256) void k_dense_cells(double h, BinState bs)
     const T inv_h = T(1.0 / h);
     for (int b = blockIdx.x; b < nbig; b += gridDim.x) {
         const int c = bs.big[b];
-        const int st = bs.start[c], n = bs.start[c + 1] - st;
-        const int seg = (n + 255) / 256;
-        const int j0 = st + min(n, t * seg), j1 = st + min(n, (t + 1) * seg);
+        const int st = bs.start[c], n = bs.start[c + 1] - st;   // n > SMALL_CELL
+        const bool sorted = n <= MAX_SORTED_CELL;
+        if (sorted) {
+            int P = 1;
+            while (P < n) P <<= 1;
+            for (int i = t; i < P; i += DENSE_THREADS) s[i] = i < n ? bs.idx[st + i] : INT_MAX;
+            __syncthreads();
+            for (int k = 2; k <= P; k <<= 1) {
+                for (int j = k >> 1; j > 0; j >>= 1) {
+                    for (int i = t; i < P; i += DENSE_THREADS) {
+                        const int l = i ^ j;
+                        if (l > i) {
+                            const int x = s[i], y = s[l];
+                            if ((x > y) == ((i & k) == 0)) {
+                                s[i] = y;
+                                s[l] = x;
+                            }
+                        }
+                    }
+                    __syncthreads();
+                }
+            }
+            for (int i = t; i < n; i += DENSE_THREADS) bs.idx[st + i] = s[i];
+        }
+        const int seg = (n + DENSE_THREADS - 1) / DENSE_THREADS;
+        const int j0 = min(n, t * seg), j1 = min(n, (t + 1) * seg);
         T acc[DENSE_VALUES];
 #pragma unroll
         for (int k = 0; k < DENSE_VALUES; ++k) acc[k] = T(0);
         for (int j = j0; j < j1; ++j) {
-            const DepRec<T> d = rec[bs.idx[j]];
+            const DepRec<T> d = rec[sorted ? s[j] : bs.idx[st + j]];
             T wx[5], wy[5];
             weights4<T>(d.xl, wx);
             weights4<T>(d.yl, wy);
@@ -513,8 +508,7 @@ hipError_t launch_deposit(const PushGeom &g, const SortArrays<TP> &a, bool use_f
         k_order_cells<<<(ncell + 255) / 256, 256, 0, s>>>(ncell, bs);
         k_sort_records<<<(int)std::min<int64_t>(8192, (g.n * rec_words + 255) / 256), 256, 0, s>>>(
             ncell, bs, rec_words);
-        k_order_big<<<256, BIG_SORT_THREADS, 0, s>>>(bs);
-        k_dense_cells<TP><<<512, 256, 0, s>>>(g.h, bs);
+        k_dense_cells<TP><<<512, DENSE_THREADS, 0, s>>>(g.h, bs);
     } else {
         // Empty sort: every cell list is empty (and no cell is dense).
         const int nb = (ncell + 1 + SCAN_TILE - 1) / SCAN_TILE;
