@@ -65,7 +65,8 @@ __device__ __forceinline__ double solver_factor(int f, int gr, int gc, int N, co
     return ok ? (1.0 / (sg.lam[gr] + sg.lam[gc] + cc)) / sg.d2 : 0.0;
 }
 
-constexpr int GBM = 64, GBN = 64;
+constexpr int GBM = 64, GBN = 64;   // output tile
+constexpr int GTJ = GBN / 32;       // 16-column products per wave (waves 2 x 2)
 // Depth of a k tile: 64 in float32, 32 in float64 (two blocks per CU fit
 // the double-buffered LDS tiles either way).
 template <typename T> struct GemmTile {
@@ -78,25 +79,26 @@ template <typename T> struct GemmTile {
 };
 
 // C_f = A_f B_f for the field f = blockIdx.z; all matrices N x N row-major.
-// Over k < M = N - 1 the products run on the matrix cores in 64 x 64 output
+// Over k < M = N - 1 the products run on the matrix cores in 64 x GBN output
 // tiles covering [0, M) (rounded up to whole tiles); the k = M term is added
-// in the epilogue.  When M is a multiple of 64 the last row and column (index
-// M) lie just outside the tiles: the blocks of the last tile row also load
-// row M of A into LDS with each k tile and accumulate row M of C for their
-// 64 columns on the vector units from the same B tile (likewise the last tile
-// column for column M, and the corner block for C[M][M]).  Otherwise row and
-// column M lie inside the last tiles.  256 threads = 4 waves; a 64 x 64 tile
-// is split 2 x 2 between the waves, each computing 32 x 32 as 2 x 2 16x16x4
-// matrix-core products.  Tiles of A (64 x BK, row-major) and B (stored
-// transposed, 64 x BK) pass through double-buffered static LDS, one barrier
-// per tile (depth BK, GemmTile); a tile's global loads are issued two tiles
-// before it is multiplied (two register sets).  Within a tile, step s (0..KS-1,
-// KS = BK / 4) of a wave's products takes k = KS lk + s for lane group
-// lk = lane / 16, the same for A and B, so each lane reads its KS A (and B)
+// in the epilogue.  Where M is a multiple of the tile size the last row
+// (column) of index M lies just outside the tiles: the blocks of the last
+// tile row also load row M of A into LDS with each k tile and accumulate row
+// M of C for their columns on the vector units from the same B tile
+// (likewise the last tile column for column M, and the corner block for
+// C[M][M]).  Otherwise row (column) M lies inside the last tiles.
+// 256 threads = 4 waves; a 64 x GBN tile is split 2 x 2 between the waves,
+// each computing 32 x GBN/2 as 2 x GBN/32 16x16x4 matrix-core products.
+// Tiles of A (64 x BK, row-major) and B (stored transposed, GBN x BK) pass
+// through double-buffered static LDS, one barrier per tile (depth BK,
+// GemmTile); a tile's global loads are issued two tiles before it is
+// multiplied (two register sets).  Within a tile, step s (0..KS-1, KS =
+// BK / 4) of a wave's products takes k = KS lk + s for lane group lk =
+// lane / 16, the same for A and B, so each lane reads its KS A (and B)
 // values of the tile as 16-byte LDS loads; the edge sums take the k slice
 // KS (tid / 64) .. + KS - 1 per thread and add the four slices in a fixed
-// order.  With SCALE the result is
-// multiplied by the solver matrix of the field (solver_factor).
+// order.  With SCALE the result is multiplied by the solver matrix of the
+// field (solver_factor).
 template <typename T, bool SCALE>
 __global__ __launch_bounds__(256, 2) void k_gemm_mfma(GemmBatch<T> bt, int N, ScaleGeom sg)
 {
@@ -118,16 +120,17 @@ __global__ __launch_bounds__(256, 2) void k_gemm_mfma(GemmBatch<T> bt, int N, Sc
     const bool xrow = M % GBM == 0 && row0 + GBM == M;   // this block also sums row M
     const bool xcol = M % GBN == 0 && col0 + GBN == M;   // ... and / or column M
     const int tid = threadIdx.x, lane = tid % 64, wave = tid / 64;
-    const int wr = (wave / 2) * 32, wc = (wave % 2) * 32;   // the wave's 32 x 32 in the tile
+    const int wr = (wave / 2) * 32, wc = (wave % 2) * (GBN / 2);   // the wave's 32 x GBN/2
     const int li = lane % 16, lk = lane / 16;
-    // Loader coordinates: A rows ar + GRA q, column ak; B rows bk + 4 q, column bc.
+    // Loader coordinates: A rows ar + GRA q, column ak; B rows bk + GRB q, column bc.
     const int ar = tid / GBK, ak = tid % GBK;
+    constexpr int GRB = 256 / GBN;
     const int bk = tid / GBN, bc = tid % GBN;
-    acc_t acc[2][2];
+    acc_t acc[2][GTJ];
 #pragma unroll
     for (int ti = 0; ti < 2; ++ti)
 #pragma unroll
-        for (int tj = 0; tj < 2; ++tj)
+        for (int tj = 0; tj < GTJ; ++tj)
 #pragma unroll
             for (int r = 0; r < 4; ++r) acc[ti][tj][r] = T(0);
     T er = 0, ec = 0, ex = 0;   // row M / column M partial sums, corner
@@ -144,7 +147,7 @@ __global__ __launch_bounds__(256, 2) void k_gemm_mfma(GemmBatch<T> bt, int N, Sc
         }
 #pragma unroll
         for (int q = 0; q < GLB; ++q) {
-            const int gk = k0 + bk + 4 * q, gc = col0 + bc;
+            const int gk = k0 + bk + GRB * q, gc = col0 + bc;
             rb[S][q] = (gk < M && gc < N) ? B[(size_t)gk * N + gc] : T(0);
         }
         if (xrow && tid < GBK) xv[S] = k0 + tid < M ? A[(size_t)M * N + k0 + tid] : T(0);
@@ -158,7 +161,7 @@ __global__ __launch_bounds__(256, 2) void k_gemm_mfma(GemmBatch<T> bt, int N, Sc
 #pragma unroll
         for (int q = 0; q < GLA; ++q) As[S][ar + GRA * q][ak] = ra[S][q];
 #pragma unroll
-        for (int q = 0; q < GLB; ++q) Bt[S][bc][bk + 4 * q] = rb[S][q];
+        for (int q = 0; q < GLB; ++q) Bt[S][bc][bk + GRB * q] = rb[S][q];
         if (xrow && tid < GBK) Xa[S][tid] = xv[S];
         if (xcol && tid >= 64 && tid < 64 + GBK) Xb[S][tid - 64] = xv[S];
     };
@@ -171,13 +174,13 @@ __global__ __launch_bounds__(256, 2) void k_gemm_mfma(GemmBatch<T> bt, int N, Sc
         if (t + 2 < nt) load(par, (t + 2) * GBK);
 #pragma unroll
         for (int s4 = 0; s4 < KS; s4 += 4) {   // four steps per 16-byte LDS read
-            T a[2][4], b[2][4];
+            T a[2][4], b[GTJ][4];
 #pragma unroll
             for (int ti = 0; ti < 2; ++ti)
 #pragma unroll
                 for (int s = 0; s < 4; ++s) a[ti][s] = As[P][wr + 16 * ti + li][KS * lk + s4 + s];
 #pragma unroll
-            for (int tj = 0; tj < 2; ++tj)
+            for (int tj = 0; tj < GTJ; ++tj)
 #pragma unroll
                 for (int s = 0; s < 4; ++s) b[tj][s] = Bt[P][wc + 16 * tj + li][KS * lk + s4 + s];
 #pragma unroll
@@ -185,10 +188,10 @@ __global__ __launch_bounds__(256, 2) void k_gemm_mfma(GemmBatch<T> bt, int N, Sc
 #pragma unroll
                 for (int ti = 0; ti < 2; ++ti)
 #pragma unroll
-                    for (int tj = 0; tj < 2; ++tj)
+                    for (int tj = 0; tj < GTJ; ++tj)
                         acc[ti][tj] = MM::mma(a[ti][s], b[tj][s], acc[ti][tj]);
         }
-        if (xrow) {
+        if (xrow && lane < GBN) {
 #pragma unroll
             for (int s = 0; s < KS; ++s) er += Xa[P][KS * wave + s] * Bt[P][lane][KS * wave + s];
         }
@@ -217,7 +220,7 @@ __global__ __launch_bounds__(256, 2) void k_gemm_mfma(GemmBatch<T> bt, int N, Sc
 #pragma unroll
     for (int ti = 0; ti < 2; ++ti)
 #pragma unroll
-        for (int tj = 0; tj < 2; ++tj)
+        for (int tj = 0; tj < GTJ; ++tj)
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
                 const int gr = row0 + wr + 16 * ti + MM::row(lk, r);
@@ -235,14 +238,14 @@ __global__ __launch_bounds__(256, 2) void k_gemm_mfma(GemmBatch<T> bt, int N, Sc
         __syncthreads();
         if (tid < GBM) {
             const T amm = A[(size_t)M * N + M];
-            if (xrow) {
+            if (xrow && tid < GBN) {
                 const int gc = col0 + tid;
                 T v = ((red[0][0][tid] + red[0][1][tid]) + (red[0][2][tid] + red[0][3][tid])) +
                       amm * B[(size_t)M * N + gc];
                 if (SCALE) v = (T)((double)v * solver_factor(f, M, gc, N, sg));
                 C[(size_t)M * N + gc] = v;
             }
-            if (xcol) {
+            if (xcol && row0 + tid < N) {   // the row tile may extend past N
                 const int gr = row0 + tid;
                 T v = ((red[1][0][tid] + red[1][1][tid]) + (red[1][2][tid] + red[1][3][tid])) +
                       A[(size_t)gr * N + M] * B[(size_t)M * N + M];
@@ -261,9 +264,10 @@ __global__ __launch_bounds__(256, 2) void k_gemm_mfma(GemmBatch<T> bt, int N, Sc
 template <typename T, bool SCALE>
 void gemm_launch(const GemmBatch<T> &bt, int N, const ScaleGeom &sg, hipStream_t s)
 {
-    const int tiles = (N - 1 + GBM - 1) / GBM;
     static_assert(GemmTile<T>::BK <= 64, "the edge loaders use threads 0..127");
-    k_gemm_mfma<T, SCALE><<<dim3(tiles, tiles, 7), 256, 0, s>>>(bt, N, sg);
+    static_assert(GBN % 32 == 0 && GBN <= 64 && GBM == 64, "wave layout 2 x 2");
+    const dim3 grid((N - 1 + GBN - 1) / GBN, (N - 1 + GBM - 1) / GBM, 7);
+    k_gemm_mfma<T, SCALE><<<grid, 256, 0, s>>>(bt, N, sg);
 }
 
 // One wave per (row, field): outputs of the row and, for Bz, its row sum.
