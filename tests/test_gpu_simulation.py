@@ -67,6 +67,45 @@ def test_on_axis_ez_matches_oracle_every_step():
     assert err < 1e-6
 
 
+def test_fp32_bulk_form_tracks_oracle():
+    """The same beam case in the FP32-bulk form (momenta, q, m, currents,
+    half-step fields and the solve in float32; positions, pz and full fields
+    in float64): the on-axis Ez of every xi step within 1e-4 of the float64
+    oracle's (normalised by max |Ez|, the bound the FP32-bulk form is held
+    to), the final fields within 1e-3 and the electron offsets within 1e-4."""
+    from lcodepy_amd.beam_generator import generate_beam
+    from lcodepy_amd.simulation import Simulation
+    cfg = fixture_config('beam_evol')
+    cfg.update({'window-width': 3.2, 'transverse-step': 0.05, 'time-limit': 1})
+    beam_params = {'current': 0.5, 'particles_in_layer': 2000}
+    c = adjusted(cfg)
+    p = O.Params.from_config(c)
+    beam = generate_beam(c, beam_params)
+    particles_o, fields_o, _, _, ez_oracle = O.simulate(p, beam, 1)
+    cfg32 = dict(cfg)
+    cfg32['precision'] = 'fp32-bulk'
+    print(f'fp32-bulk N={p.N}', flush=True)
+    sim = Simulation(config=cfg32, beam_parameters=beam_params, runas_filename='')
+    sim.push_solver.verbose = False
+    sim.step()
+    ez_gpu = np.array([float(v) for v in sim.push_solver.ez_axis])
+    assert ez_gpu.shape == ez_oracle.shape
+    err = np.max(np.abs(ez_gpu - ez_oracle)) / np.max(np.abs(ez_oracle))
+    particles, fields, _ = sim.push_solver._plasmastate
+    worst = {'ez_axis': err}
+    for k in ('Ex', 'Ey', 'Ez', 'Bx', 'By', 'Bz'):
+        worst[k] = relerr(getattr(fields, k).double().cpu().numpy(), fields_o[k])
+    el, el_o = particles['electrons'], particles_o['electrons']
+    for k in ('x_offt', 'y_offt'):
+        worst[k] = relerr(getattr(el, k).cpu().numpy(), el_o[k])
+    print(f'fp32-bulk N={p.N}, {len(ez_gpu)} xi steps:', {k: f'{v:.1e}' for k, v in worst.items()})
+    assert err < 1e-4
+    for k in ('Ex', 'Ey', 'Ez', 'Bx', 'By', 'Bz'):
+        assert worst[k] < 1e-3, (k, worst[k])
+    for k in ('x_offt', 'y_offt'):
+        assert worst[k] < 1e-4, (k, worst[k])
+
+
 def test_beam_evolution_matches_oracle():
     """Three time steps of a 65 x 65 window with a generated beam: the beam
     drained after the last time step (every particle's id, position and
