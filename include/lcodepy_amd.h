@@ -14,6 +14,7 @@
  *   lc3d_deposit_beam     beam3d/weights.py:152-163  (deposit_beam)
  *   lc3d_beam_substep     beam3d/beam_calculator.py:97-114 (start_moving_layer)
  *   lc3d_move_beam        beam3d/move.py:294-309     (move_beam_particles)
+ *   lc3d_energy_flux      diagnostics/FXi/Strategy3D.py:6-25 (calculate_energy_fluxes)
  *
  * Conventions
  *   - All pointers are device pointers (hipMalloc'd or torch-ROCm tensors),
@@ -196,6 +197,17 @@ int lc3d_move_beam(lc3d_ctx *ctx, int64_t n, int64_t beam_layer_idx, const doubl
                    int64_t *remaining_steps, int64_t *id, double *x, double *y, double *xi,
                    double *px, double *py, double *pz, uint8_t *lost, uint8_t *moved,
                    uint8_t *fell, void *stream);
+
+/* Total energy flux of a plasma layer, diagnostics/FXi/Strategy3D.py:6-25
+ * (calculate_energy_fluxes): the sum over the grid of (Ez^2 + Bz^2 +
+ * (Ex - By)^2 + (Ey + Bx)^2) h^2 / 2 plus the sum over the n electrons of
+ * sqrt(m^2 + px^2 + py^2 + pz^2) - m, with h the context's transverse step.
+ * fields: the float64 (N, N) Ex..Bz; m, px, py of the bulk type, pz float64.
+ * Writes one float64 to *psi (device memory), in a fixed summation order,
+ * without synchronising the host. */
+int lc3d_energy_flux(lc3d_ctx *ctx, const lc3d_fields6 *fields, int64_t n, const void *m,
+                     const void *px, const void *py, const double *pz, double *psi,
+                     void *stream);
 
 #ifdef __cplusplus
 }

@@ -73,7 +73,7 @@ class Sort(ctypes.Structure):
 SYMBOLS = ('lc3d_create', 'lc3d_destroy', 'lc3d_last_error', 'lc3d_precision',
            'lc3d_error_flags', 'lc3d_kernel_timing', 'lc3d_kernel_times', 'lc3d_move_wo_fields', 'lc3d_move_smart', 'lc3d_deposit',
            'lc3d_compute_fields', 'lc3d_plasma_step', 'lc3d_deposit_beam',
-           'lc3d_beam_substep', 'lc3d_move_beam')
+           'lc3d_beam_substep', 'lc3d_move_beam', 'lc3d_energy_flux')
 
 _LIB = None
 _LOCK = threading.Lock()
@@ -106,6 +106,7 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
                                    P(Fields), P(Fields), P(Currents), _vp]
     L.lc3d_deposit_beam.argtypes = [_vp, i64, i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp]
     L.lc3d_beam_substep.argtypes = [_vp, i64, _vp, _vp, dbl, dbl, _vp, _vp, _vp]
+    L.lc3d_energy_flux.argtypes = [_vp, P(Fields6), i64, _vp, _vp, _vp, _vp, _vp, _vp]
     L.lc3d_move_beam.argtypes = [_vp, i64, i64, _vp, _vp, P(Fields6), P(Fields6), _vp, _vp,
                                  _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]
     for name in SYMBOLS:
@@ -151,6 +152,7 @@ class Context:
             raise RuntimeError('lcodepy_amd needs a ROCm GPU (torch.cuda.is_available() is False)')
         self.lib = lib()
         self.N = N
+        self.h = float(h)
         self.precision = precision
         self.max_particles = int(max_particles)
         p = Params()

@@ -13,6 +13,9 @@
 
 using namespace lc3d;
 
+// Blocks of the energy-flux reduction (each leaves two float64 partials).
+constexpr int DIAG_BLOCKS = 1024;
+
 struct lc3d_ctx {
     lc3d_params prm{};
     int N = 0;
@@ -35,6 +38,7 @@ struct lc3d_ctx {
     int solver = LC3D_SOLVER_MATRIX;
     void *work = nullptr, *work2 = nullptr;
     double *rowsum = nullptr;
+    double *diag = nullptr;   // energy-flux partial sums, 2 DIAG_BLOCKS
     void *tw = nullptr;                                      // FFT twiddles
     void *tdst = nullptr, *tdct = nullptr, *tdct_t = nullptr;   // transform matrices
     double *lam = nullptr;
@@ -386,6 +390,7 @@ int lc3d_create(const lc3d_params *p, lc3d_ctx **out)
               hipMalloc(&c->dense, dense_cell_capacity(cap) * 100 * tg) == hipSuccess &&
               hipMalloc(&c->work, 7 * nn * tg) == hipSuccess &&
               hipMalloc(&c->rowsum, N * sizeof(double)) == hipSuccess &&
+              hipMalloc(&c->diag, 2 * DIAG_BLOCKS * sizeof(double)) == hipSuccess &&
               hipMalloc(&c->lam, N * sizeof(double)) == hipSuccess;
     if (ok && fft) ok = hipMalloc(&c->tw, 2 * (size_t)M * 2 * tg) == hipSuccess;
     if (ok && !fft)
@@ -481,6 +486,7 @@ void lc3d_destroy(lc3d_ctx *c)
     (void)hipFree(c->work);
     (void)hipFree(c->work2);
     (void)hipFree(c->rowsum);
+    (void)hipFree(c->diag);
     (void)hipFree(c->tw);
     (void)hipFree(c->tdst);
     (void)hipFree(c->tdct);
@@ -690,6 +696,29 @@ int lc3d_move_beam(lc3d_ctx *c, int64_t n, int64_t beam_layer_idx, const double 
                                             lost, moved, fell, c->err,
                                             static_cast<hipStream_t>(stream));
     if (e != hipSuccess) return hip_fail(c, e, "move_beam");
+    return LC3D_OK;
+}
+
+int lc3d_energy_flux(lc3d_ctx *c, const lc3d_fields6 *f, int64_t n, const void *m,
+                     const void *px, const void *py, const double *pz, double *psi, void *stream)
+{
+    if (!c || !f || !f->Ex || !f->Ey || !f->Ez || !f->Bx || !f->By || !f->Bz || !psi || n < 0 ||
+        (n > 0 && (!m || !px || !py || !pz)))
+        return fail(c, LC3D_EINVAL, "energy_flux: bad arguments");
+    const double h = c->prm.grid_step_size;
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    hipError_t e;
+    if (fp32(c))
+        e = launch_energy_flux<float>(c->N, h, f6<double>(f), n, static_cast<const float *>(m),
+                                      static_cast<const float *>(px),
+                                      static_cast<const float *>(py), pz, c->diag, DIAG_BLOCKS,
+                                      psi, st);
+    else
+        e = launch_energy_flux<double>(c->N, h, f6<double>(f), n, static_cast<const double *>(m),
+                                       static_cast<const double *>(px),
+                                       static_cast<const double *>(py), pz, c->diag, DIAG_BLOCKS,
+                                       psi, st);
+    if (e != hipSuccess) return hip_fail(c, e, "energy_flux");
     return LC3D_OK;
 }
 

@@ -200,6 +200,14 @@ hipError_t launch_compute_fields_matrix(const MatrixGeom &g, const SolveIn<TG, T
                                         const SolveOut<TFULL, THALF> &out, bool write_full,
                                         hipStream_t s, KernelTimer *timer = nullptr);
 
+// ---- diagnostics ---------------------------------------------------------------
+// diagnostics/FXi/Strategy3D.py:6-25: total energy flux of a plasma layer into
+// psi[0]; part holds 2 max_blocks float64 partial sums.
+template <typename TP>
+hipError_t launch_energy_flux(int N, double h, const Fields6<double> &f, int64_t n, const TP *m,
+                              const TP *px, const TP *py, const double *pz, double *part,
+                              int max_blocks, double *psi, hipStream_t s);
+
 // ---- beam (beam3d) -----------------------------------------------------------
 struct BeamGeom {
     int N;

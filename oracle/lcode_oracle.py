@@ -18,11 +18,14 @@ It restates, in numpy plus the C kernels of ``oracle/kernels.c``:
                                    beam3d/beam_io.py:7-66, beam3d/data.py:88-169
 * the xi loop                   -- push_solvers/push_solver.py:215-286
 * the time loop                 -- simulation.py:244-264, mpi/beam_io.py:81-119
+* the energy flux diagnostic    -- diagnostics/FXi/Strategy3D.py:6-25
 
 Parity pins (tests/test_oracle_*.py): the reference's own end-state fixtures
 ``3D_test1.npz`` and ``3D_beam_evol.npz`` (tests/integration/test_3D_part.py,
 rtol 5e-8), closed-form eigenmodes of the three solves, DST-I/DCT-I
-self-inverse identities, spline-weight invariants and single-particle cases.
+self-inverse identities, spline-weight invariants and single-particle cases;
+the energy flux by closed forms (uniform fields, the cancelling Ex = By,
+Ey = -Bx combination, a single particle).
 """
 from __future__ import annotations
 
@@ -563,3 +566,20 @@ def simulate(p: Params, beam_records: np.ndarray, n_time_steps: int, electrons=N
         if t_i != n_time_steps - 1:
             source = beam_sorted(out[3])
     return out
+
+
+# --------------------------------------------------------------------------
+# diagnostics
+# --------------------------------------------------------------------------
+
+def energy_flux(h: float, electrons: dict, fields: dict) -> float:
+    """Total energy flux of a plasma layer (diagnostics/FXi/Strategy3D.py:6-25):
+    the electromagnetic sum of (Ez^2 + Bz^2 + (Ex - By)^2 + (Ey + Bx)^2) h^2 / 2
+    over the grid plus the electrons' motion energy sum of
+    sqrt(m^2 + px^2 + py^2 + pz^2) - m."""
+    f = {k: np.asarray(fields[k], dtype=np.float64) for k in ('Ex', 'Ey', 'Ez', 'Bx', 'By', 'Bz')}
+    se = (f['Ez'] ** 2 + f['Bz'] ** 2 + (f['Ex'] - f['By']) ** 2 +
+          (f['Ey'] + f['Bx']) ** 2) * h ** 2 / 2
+    e = {k: np.asarray(electrons[k], dtype=np.float64) for k in ('m', 'px', 'py', 'pz')}
+    gamma_m = np.sqrt(e['m'] ** 2 + e['px'] ** 2 + e['py'] ** 2 + e['pz'] ** 2)
+    return float(np.sum(gamma_m - e['m']) + np.sum(se))

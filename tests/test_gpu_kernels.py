@@ -244,3 +244,36 @@ def test_plasma_step_matches_oracle(N, ion_model):
     for k in ('ro', 'jx', 'jy', 'jz'):
         assert relerr(host(getattr(cur_g, k)), cur_o[k]) < 1e-9, k
     print(f'worst field error after 30 steps: {worst:.2e}')
+
+
+@pytest.mark.parametrize('precision', ['fp64', 'fp32-bulk'])
+def test_energy_flux_matches_oracle(precision):
+    """calculate_energy_fluxes (Strategy3D.py:6-25) on the device against the
+    oracle's restatement on the same state (10 plasma steps with a beam at
+    N = 65, so fields and momenta are non-trivial): within 1e-12, the inputs
+    being identical and only the summation order differing."""
+    from lcodepy_amd.diagnostics import calculate_energy_fluxes
+    from lcodepy_amd.plasma3d.initialization import init_plasma
+    from lcodepy_amd.plasma3d.solver import Plane2d3vPlasmaSolver
+    N = 65
+    print(f'energy flux N={N} {precision}', flush=True)
+    cfg = _cfg(N, precision=precision)
+    h = cfg.getfloat('transverse-step')
+    fields, parts, cur, const, _ = init_plasma(cfg)
+    solver = Plane2d3vPlasmaSolver(cfg)
+    rb_prev = np.zeros((N, N))
+    for step in range(10):
+        rb_full = _gauss_beam(N, h, -(step + 1) * 0.05, amp=-0.5)
+        parts, fields, cur = solver.step_dxi(parts, fields, cur, const, dev(rb_full), dev(rb_prev))
+        rb_prev = rb_full
+    psi = calculate_energy_fluxes(h, parts, fields)
+    torch.cuda.synchronize()
+    solver.ctx.raise_on_errors()
+    el = parts['electrons']
+    ref = O.energy_flux(h, {k: host(getattr(el, k)) for k in ('m', 'px', 'py', 'pz')},
+                        {k: host(getattr(fields, k)) for k in ('Ex', 'Ey', 'Ez', 'Bx', 'By', 'Bz')})
+    got = float(psi.item())
+    print(f'energy flux {precision}: {got:.15e} vs oracle {ref:.15e}')
+    assert psi.dtype == torch.float64 and psi.is_cuda
+    assert ref != 0.0
+    assert abs(got - ref) <= 1e-12 * abs(ref)

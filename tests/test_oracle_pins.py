@@ -311,3 +311,48 @@ def test_beam_substep_multiplier():
     for d, z in zip(dt, pz):
         max_dt = math.sqrt(math.sqrt(1 + z * z) / 2.0)
         assert d <= max_dt and (d == 1.0 or 2 * d > max_dt)
+
+
+# ---- energy flux (diagnostics/FXi/Strategy3D.py:6-25) ----------------------------
+
+def _zero_fields(N):
+    return {k: np.zeros((N, N)) for k in ('Ex', 'Ey', 'Ez', 'Bx', 'By', 'Bz')}
+
+
+def _rest(n, m=0.25):
+    return {'m': np.full(n, m), 'px': np.zeros(n), 'py': np.zeros(n), 'pz': np.zeros(n)}
+
+
+def test_energy_flux_zero_state():
+    assert O.energy_flux(0.1, _rest(10), _zero_fields(7)) == 0.0
+
+
+def test_energy_flux_uniform_fields():
+    # Uniform Ez = a, Bz = b over N^2 cells: N^2 (a^2 + b^2) h^2 / 2.
+    N, h, a, b = 9, 0.05, 0.3, -0.7
+    f = _zero_fields(N)
+    f['Ez'][:] = a
+    f['Bz'][:] = b
+    assert np.isclose(O.energy_flux(h, _rest(4), f), N * N * (a * a + b * b) * h * h / 2,
+                      rtol=1e-14)
+
+
+def test_energy_flux_transverse_combination_cancels():
+    # Ex = By and Ey = -Bx carry no flux; Ex = -By doubles it: (2c)^2 h^2 / 2 per cell.
+    rng = np.random.default_rng(5)
+    N, h = 11, 0.1
+    f = _zero_fields(N)
+    c = rng.normal(size=(N, N))
+    d = rng.normal(size=(N, N))
+    f['Ex'], f['By'], f['Ey'], f['Bx'] = c, c.copy(), d, -d
+    assert O.energy_flux(h, _rest(3), f) == 0.0
+    f['By'] = -c
+    assert np.isclose(O.energy_flux(h, _rest(3), f), np.sum((2 * c) ** 2) * h * h / 2,
+                      rtol=1e-13)
+
+
+def test_energy_flux_single_particle():
+    # m = 3, |p| = 4 (px, py, pz = 0, 0, 4 and 2, 2, 2*sqrt(2)): gamma m - m = 5 - 3.
+    e = {'m': np.array([3.0, 3.0]), 'px': np.array([0.0, 2.0]), 'py': np.array([0.0, 2.0]),
+         'pz': np.array([4.0, 2 * np.sqrt(2.0)])}
+    assert np.isclose(O.energy_flux(0.1, e, _zero_fields(5)), 4.0, rtol=1e-15)
