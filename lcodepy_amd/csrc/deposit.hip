@@ -315,6 +315,12 @@ template <typename T> struct alignas(4 * sizeof(T)) Tap4 {
 template <typename T> struct TapPrefetch {
     static constexpr int PF = sizeof(T) == 4 ? 4 : 2;
 };
+// Gather blocks per CU asked of the compiler: five in float32, whose 31 KB of
+// LDS taps allow it, so that the 33 x 33 tiles of a 513 x 513 grid run in one
+// round; two in float64 (62 KB).
+template <typename T> struct GatherOcc {
+    static constexpr int value = sizeof(T) == 4 ? 5 : 2;
+};
 
 template <int A, typename T>
 __device__ __forceinline__ void add_taps(const DepRec<T> &d, T inv_h, Tap4<T> tp[5])
@@ -335,14 +341,14 @@ __device__ __forceinline__ void add_taps(const DepRec<T> &d, T inv_h, Tap4<T> tp
 // The five taps toward x offset A - 2 (target cells (ix + A - 2, iy + b - 2),
 // b = 0..4) of one source cell, for the four currents: the sum over its
 // particles, in list order, of c_q (w_A(xl) / h) (w_b(yl) / h) -- the first
-// PF records from the registers `pre`, the others from rec (= rec_sorted,
-// first = the cell's list start) -- or the precomputed block of a dense cell.
-template <int A, typename T>
+// PF (0 or TapPrefetch) records from the registers `pre`, the others from rec
+// (= rec_sorted, first = the cell's list start) -- or the precomputed block
+// of a dense cell.
+template <int A, typename T, int PF>
 __device__ __forceinline__ void cell_taps(const DepRec<T> *__restrict__ rec,
                                           const T *__restrict__ dense, int first, int n, int dsl,
                                           const DepRec<T> *pre, T inv_h, Tap4<T> tp[5])
 {
-    constexpr int PF = TapPrefetch<T>::PF;
 #pragma unroll
     for (int b = 0; b < 5; ++b)
 #pragma unroll
@@ -382,16 +388,13 @@ __device__ __forceinline__ void cell_taps(const DepRec<T> *__restrict__ rec,
 // thus the same fixed-order sum of 25 taps, written by exactly one thread:
 // no floating-point atomics, reproducible bit for bit.
 template <typename TP, typename TG, bool KAHAN>
-__global__ __launch_bounds__(256) void k_deposit_gather(
+__global__ __launch_bounds__(256, GatherOcc<TP>::value) void k_deposit_gather(
     int N, double h, const int *__restrict__ start, BinState bs, TG *__restrict__ ro,
     TG *__restrict__ jx, TG *__restrict__ jy, TG *__restrict__ jz, const TG *__restrict__ bg_ro,
     TG *__restrict__ ro1, TG *__restrict__ jx1, TG *__restrict__ jy1, TG *__restrict__ jz1)
 {
     using T = TP;   // arithmetic type of the taps
     constexpr int H = DEP_HALO;
-    __shared__ int s_first[H * H];   // list position of each halo cell's particles
-    __shared__ int s_cnt[H * H];     // their number
-    __shared__ int s_dsl[H * H];     // its dense slot, or -1
     __shared__ Tap4<T> s_tap[5][H * H];
 
     const int tile = xcd_swizzle(blockIdx.x, gridDim.x);
@@ -404,14 +407,19 @@ __global__ __launch_bounds__(256) void k_deposit_gather(
     const int nrow = hx1 - hx0, ncol = hy1 - hy0, nc = nrow * ncol;
     const int t = threadIdx.x;
 
-    for (int i = t; i < nc; i += 256) {
-        const int cell = (hx0 + i / ncol) * N + hy0 + i % ncol;
-        const int st = start[cell];
-        s_first[i] = st;
-        s_cnt[i] = start[cell + 1] - st;
-        s_dsl[i] = bs.dslot[cell];
+    // This thread's halo cells, t and t + 256 (nc <= 400): list start,
+    // particle count and dense slot (or -1).
+    int c_first[2] = {0, 0}, c_cnt[2] = {0, 0}, c_dsl[2] = {-1, -1};
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+        const int i = t + 256 * u;
+        if (i < nc) {
+            const int cell = (hx0 + i / ncol) * N + hy0 + i % ncol;
+            c_first[u] = start[cell];
+            c_cnt[u] = start[cell + 1] - c_first[u];
+            c_dsl[u] = bs.dslot[cell];
+        }
     }
-    __syncthreads();
 
     const int tx = t / DEP_TILE, ty = t % DEP_TILE;
     const int ox = X0 + tx, oy = Y0 + ty;
@@ -422,16 +430,14 @@ __global__ __launch_bounds__(256) void k_deposit_gather(
 
     // This thread's halo cells (t and t + 256; nc <= 400) and their first
     // records, loaded once for the five passes.
+    // (Only the first cell's: a thread's second cell, t + 256 < nc, reads its
+    // records in each pass.)
     constexpr int PF = TapPrefetch<T>::PF;
-    DepRec<T> pre[2][PF];
+    DepRec<T> pre[PF];
+    if (t < nc && c_dsl[0] < 0) {
 #pragma unroll
-    for (int u = 0; u < 2; ++u) {
-        const int i = t + 256 * u;
-        if (i < nc && s_dsl[i] < 0) {
-#pragma unroll
-            for (int k = 0; k < PF; ++k)
-                if (k < s_cnt[i]) pre[u][k] = rec[s_first[i] + k];
-        }
+        for (int k = 0; k < PF; ++k)
+            if (k < c_cnt[0]) pre[k] = rec[c_first[0] + k];
     }
 
     auto pass = [&](auto a_const) {
@@ -441,7 +447,11 @@ __global__ __launch_bounds__(256) void k_deposit_gather(
             const int i = t + 256 * u;
             if (i < nc) {
                 Tap4<T> tp[5];
-                cell_taps<A, T>(rec, dense, s_first[i], s_cnt[i], s_dsl[i], pre[u], inv_h, tp);
+                if (u == 0)
+                    cell_taps<A, T, TapPrefetch<T>::PF>(rec, dense, c_first[0], c_cnt[0], c_dsl[0],
+                                                        pre, inv_h, tp);
+                else
+                    cell_taps<A, T, 0>(rec, dense, c_first[1], c_cnt[1], c_dsl[1], pre, inv_h, tp);
 #pragma unroll
                 for (int b = 0; b < 5; ++b) s_tap[b][i] = tp[b];
             }
