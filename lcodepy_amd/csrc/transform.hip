@@ -15,7 +15,7 @@
 // interior points and return exact zeros on the edges.
 //
 // Kernels, all on plain global buffers and static LDS:
-//   k_rhs       R of all seven fields (one thread per grid point);
+//   k_rhs       R of all seven fields (one thread per grid point, all fields);
 //   k_gemm_mfma C_f = A_f B_f for the seven fields of a launch (64 x 64
 //               tiles through LDS, 16x16x4 matrix-core products; the last
 //               row and column by the blocks of the last tile row and
@@ -40,16 +40,19 @@ struct ScaleGeom {
     double d2;     // (2 (N - 1))^2
 };
 
+// One thread per grid point evaluates the seven fields' right-hand sides
+// there (the field loop unrolled, so the currents they share are loaded
+// once).
 template <typename TG, typename TPRED>
 __global__ __launch_bounds__(256) void k_rhs(RhsEval<TG, TPRED> rhs, TG *__restrict__ work)
 {
     const int N = rhs.N;
-    const int f = blockIdx.y;
     const int64_t nn = (int64_t)N * N;
     const int64_t o = blockIdx.x * (int64_t)256 + threadIdx.x;
     if (o >= nn) return;
     const int i = (int)(o / N), j = (int)(o % N);
-    work[f * nn + o] = in_rhs_domain(f, i, j, N) ? rhs(f, i, j) : TG(0);
+#pragma unroll
+    for (int f = 0; f < 7; ++f) work[f * nn + o] = in_rhs_domain(f, i, j, N) ? rhs(f, i, j) : TG(0);
 }
 
 // The solver-matrix factor of field f at spectral indices (kx, ky) = (gr, gc)
@@ -319,7 +322,7 @@ hipError_t launch_compute_fields_matrix(const MatrixGeom &g, const SolveIn<TG, T
         ix.A[f] = Tx;          ix.B[f] = w1 + f * nn; ix.C[f] = w2 + f * nn;
         iy.A[f] = w2 + f * nn; iy.B[f] = Ty_t; iy.C[f] = w1 + f * nn;
     }
-    k_rhs<TG, TPRED><<<dim3((unsigned)((nn + 255) / 256), 7), 256, 0, s>>>(rhs, w1);
+    k_rhs<TG, TPRED><<<(unsigned)((nn + 255) / 256), 256, 0, s>>>(rhs, w1);
     hipError_t e;
     if (timer && (e = timer->begin(KC_SOLVE_GEMM, 4, s)) != hipSuccess) return e;
     gemm_launch<TG, false>(fy, N, sg, s);
