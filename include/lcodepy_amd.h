@@ -15,6 +15,8 @@
  *   lc3d_beam_substep     beam3d/beam_calculator.py:97-114 (start_moving_layer)
  *   lc3d_move_beam        beam3d/move.py:294-309     (move_beam_particles)
  *   lc3d_energy_flux      diagnostics/FXi/Strategy3D.py:6-25 (calculate_energy_fluxes)
+ *   lc3d_probe_sample     diagnostics/FXi/Strategy3D.py:59-82 (the f(xi) probe values)
+ *   lc3d_abs_max          diagnostics/FXi/Strategy3D.py:84-95 (process_chaotic*)
  *
  * Conventions
  *   - All pointers are device pointers (hipMalloc'd or torch-ROCm tensors),
@@ -208,6 +210,27 @@ int lc3d_move_beam(lc3d_ctx *ctx, int64_t n, int64_t beam_layer_idx, const doubl
 int lc3d_energy_flux(lc3d_ctx *ctx, const lc3d_fields6 *fields, int64_t n, const void *m,
                      const void *px, const void *py, const double *pz, double *psi,
                      void *stream);
+
+/* ---- f(xi) probes ------------------------------------------------------------------ */
+#define LC3D_PROBE_MAX_SOURCES 16
+typedef struct lc3d_probe_sources {
+    int32_t n;                                  /* sources used, <= LC3D_PROBE_MAX_SOURCES */
+    const void *src[LC3D_PROBE_MAX_SOURCES];    /* (N, N) grids (a current column is one) */
+    int32_t is_f32[LC3D_PROBE_MAX_SOURCES];     /* 1: float32 grid, 0: float64 */
+} lc3d_probe_sources;
+
+/* One xi step of diagnostics/FXi/Strategy3D.py:59-82 (process_field,
+ * process_n, process_rho, process_Phi): every source at the n_points probe
+ * cells (ix[p], iy[p]) (device int32 arrays) into out[s * n_points + p]
+ * (device, float64), without synchronising; a cell off the grid reads NaN. */
+int lc3d_probe_sample(lc3d_ctx *ctx, const lc3d_probe_sources *sources, int32_t n_points,
+                      const int32_t *ix, const int32_t *iy, double *out, void *stream);
+
+/* Strategy3D.py:84-95 (process_chaotic, process_chaotic_perp): max |a| over
+ * rows [r0, r1) and columns [c0, c1) of an (nr, nc) float64 device array
+ * into *out (device), without synchronising. */
+int lc3d_abs_max(lc3d_ctx *ctx, const double *a, int64_t nr, int64_t nc, int64_t r0, int64_t r1,
+                 int64_t c0, int64_t c1, double *out, void *stream);
 
 #ifdef __cplusplus
 }

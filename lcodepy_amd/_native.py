@@ -60,6 +60,15 @@ class Currents(ctypes.Structure):
     _fields_ = [(k, _vp) for k in ('ro', 'jx', 'jy', 'jz')]
 
 
+PROBE_MAX_SOURCES = 16
+
+
+class ProbeSources(ctypes.Structure):
+    _fields_ = [('n', ctypes.c_int32),
+                ('src', _vp * PROBE_MAX_SOURCES),
+                ('is_f32', ctypes.c_int32 * PROBE_MAX_SOURCES)]
+
+
 class Sort(ctypes.Structure):
     _fields_ = [('n', ctypes.c_int64),
                 ('x_init', _vp), ('y_init', _vp), ('m', _vp),
@@ -73,7 +82,8 @@ class Sort(ctypes.Structure):
 SYMBOLS = ('lc3d_create', 'lc3d_destroy', 'lc3d_last_error', 'lc3d_precision',
            'lc3d_error_flags', 'lc3d_kernel_timing', 'lc3d_kernel_times', 'lc3d_move_wo_fields', 'lc3d_move_smart', 'lc3d_deposit',
            'lc3d_compute_fields', 'lc3d_plasma_step', 'lc3d_deposit_beam',
-           'lc3d_beam_substep', 'lc3d_move_beam', 'lc3d_energy_flux')
+           'lc3d_beam_substep', 'lc3d_move_beam', 'lc3d_energy_flux', 'lc3d_probe_sample',
+           'lc3d_abs_max')
 
 _LIB = None
 _LOCK = threading.Lock()
@@ -107,6 +117,8 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
     L.lc3d_deposit_beam.argtypes = [_vp, i64, i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp]
     L.lc3d_beam_substep.argtypes = [_vp, i64, _vp, _vp, dbl, dbl, _vp, _vp, _vp]
     L.lc3d_energy_flux.argtypes = [_vp, P(Fields6), i64, _vp, _vp, _vp, _vp, _vp, _vp]
+    L.lc3d_probe_sample.argtypes = [_vp, P(ProbeSources), i32, _vp, _vp, _vp, _vp]
+    L.lc3d_abs_max.argtypes = [_vp, _vp, i64, i64, i64, i64, i64, i64, _vp, _vp]
     L.lc3d_move_beam.argtypes = [_vp, i64, i64, _vp, _vp, P(Fields6), P(Fields6), _vp, _vp,
                                  _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]
     for name in SYMBOLS:

@@ -30,10 +30,11 @@ struct lc3d_ctx {
     int *big = nullptr, *nbig = nullptr, *dslot = nullptr;
     void *dense = nullptr;
     int bigcap = 0;
-    // idx is a permutation of the sort with this x_init and size after its
-    // last deposit; pushes of that sort walk it (cell order).
-    const void *order_x_init = nullptr;
-    int64_t order_n = -1;
+    // idx holds one list per current column (cap ints each): after a deposit
+    // of the sort with this x_init and size into column k, its part k is a
+    // permutation of the sort in cell order, which pushes of that sort walk.
+    const void *order_x_init[2] = {nullptr, nullptr};
+    int64_t order_n[2] = {-1, -1};
     // field solve
     int solver = LC3D_SOLVER_MATRIX;
     void *work = nullptr, *work2 = nullptr;
@@ -110,11 +111,12 @@ PushGeom push_geom(const lc3d_ctx *c, int64_t n)
     return PushGeom{n, c->N, c->prm.grid_step_size, c->prm.xi_step_size, c->prm.boundary};
 }
 
-BinState bin_state(const lc3d_ctx *c)
+// Binning state of a deposit into (or a push ordered by) current column col.
+BinState bin_state(const lc3d_ctx *c, int col)
 {
-    return BinState{c->count, c->start, c->blocks, c->key, c->rank, c->idx, c->err, c->rec,
-                    std::max<int64_t>(1, c->cap), c->big, c->nbig, c->bigcap, c->dslot,
-                    c->dense, c->rec_sorted};
+    const int64_t cap = std::max<int64_t>(1, c->cap);
+    return BinState{c->count, c->start, c->blocks, c->key, c->rank, c->idx + col * cap, c->err,
+                    c->rec, cap, c->big, c->nbig, c->bigcap, c->dslot, c->dense, c->rec_sorted};
 }
 
 template <typename TP> SortArrays<TP> sort_arrays(const lc3d_sort *s)
@@ -252,7 +254,7 @@ int deposit_sort(lc3d_ctx *c, const lc3d_sort *s, bool binned, const lc3d_curren
     if (fp32(c)) {
         auto at = [&](void *p, int k) { return static_cast<float *>(p) + k * nn; };
         e = launch_deposit<float, float>(push_geom(c, s->n), sort_arrays<float>(s), true, binned,
-                                         bin_state(c), at(cur->ro, col), at(cur->jx, col),
+                                         bin_state(c, col), at(cur->ro, col), at(cur->jx, col),
                                          at(cur->jy, col), at(cur->jz, col),
                                          bg ? static_cast<const float *>(bg_ro) : nullptr,
                                          at(cur->ro, 1), at(cur->jx, 1), at(cur->jy, 1),
@@ -260,22 +262,26 @@ int deposit_sort(lc3d_ctx *c, const lc3d_sort *s, bool binned, const lc3d_curren
     } else {
         auto at = [&](void *p, int k) { return static_cast<double *>(p) + k * nn; };
         e = launch_deposit<double, double>(push_geom(c, s->n), sort_arrays<double>(s), true, binned,
-                                           bin_state(c), at(cur->ro, col), at(cur->jx, col),
+                                           bin_state(c, col), at(cur->ro, col), at(cur->jx, col),
                                            at(cur->jy, col), at(cur->jz, col),
                                            bg ? static_cast<const double *>(bg_ro) : nullptr,
                                            at(cur->ro, 1), at(cur->jx, 1), at(cur->jy, 1),
                                            at(cur->jz, 1), st, timer_of(c));
     }
     if (e != hipSuccess) return hip_fail(c, e, "deposit");
-    c->order_x_init = s->x_init;
-    c->order_n = s->n;
+    c->order_x_init[col] = s->x_init;
+    c->order_n[col] = s->n;
     return LC3D_OK;
 }
 
-// The cell order of the last deposit, when it belongs to this sort.
+// The cell order of the last deposit into this sort's column, when it
+// belongs to this sort.
 const int *push_order(const lc3d_ctx *c, const lc3d_sort *s)
 {
-    return (s->n > 0 && c->order_n == s->n && c->order_x_init == s->x_init) ? c->idx : nullptr;
+    const int col = s->column;
+    return (s->n > 0 && c->order_n[col] == s->n && c->order_x_init[col] == s->x_init)
+               ? c->idx + col * std::max<int64_t>(1, c->cap)
+               : nullptr;
 }
 
 int push_sort(lc3d_ctx *c, const lc3d_sort *s, const void *const *f, bool f32, bool first,
@@ -286,16 +292,16 @@ int push_sort(lc3d_ctx *c, const lc3d_sort *s, const void *const *f, bool f32, b
     if (fp32(c)) {
         if (f32)
             e = launch_push<float, float>(push_geom(c, s->n), f6<float>(&fl), sort_arrays<float>(s),
-                                          first, bin, bin_state(c), push_order(c, s), st,
+                                          first, bin, bin_state(c, s->column), push_order(c, s), st,
                                           timer_of(c));
         else
             e = launch_push<float, double>(push_geom(c, s->n), f6<double>(&fl),
-                                           sort_arrays<float>(s), first, bin, bin_state(c),
+                                           sort_arrays<float>(s), first, bin, bin_state(c, s->column),
                                            push_order(c, s), st, timer_of(c));
     } else {
         if (f32) return fail(c, LC3D_EINVAL, "float32 fields need an FP32-bulk context");
         e = launch_push<double, double>(push_geom(c, s->n), f6<double>(&fl),
-                                        sort_arrays<double>(s), first, bin, bin_state(c),
+                                        sort_arrays<double>(s), first, bin, bin_state(c, s->column),
                                         push_order(c, s), st, timer_of(c));
     }
     if (e != hipSuccess) return hip_fail(c, e, "push");
@@ -380,7 +386,7 @@ int lc3d_create(const lc3d_params *p, lc3d_ctx **out)
               hipMalloc(&c->blocks, nblocks * sizeof(int)) == hipSuccess &&
               hipMalloc(&c->key, cap * sizeof(int)) == hipSuccess &&
               hipMalloc(&c->rank, cap * sizeof(int)) == hipSuccess &&
-              hipMalloc(&c->idx, cap * sizeof(int)) == hipSuccess &&
+              hipMalloc(&c->idx, 2 * cap * sizeof(int)) == hipSuccess &&
               hipMalloc(&c->err, sizeof(unsigned)) == hipSuccess &&
               hipMalloc(&c->rec, cap * deposit_record_bytes(f32)) == hipSuccess &&
               hipMalloc(&c->rec_sorted, cap * deposit_record_bytes(f32)) == hipSuccess &&
@@ -584,14 +590,14 @@ int lc3d_deposit(lc3d_ctx *c, const lc3d_sort *s, void *ro, void *jx, void *jy, 
     hipError_t e;
     if (fp32(c))
         e = launch_deposit<float, float>(push_geom(c, s->n), sort_arrays<float>(s), true, false,
-                                         bin_state(c), static_cast<float *>(ro),
+                                         bin_state(c, s->column), static_cast<float *>(ro),
                                          static_cast<float *>(jx), static_cast<float *>(jy),
                                          static_cast<float *>(jz), static_cast<const float *>(bg_ro),
                                          static_cast<float *>(ro1), static_cast<float *>(jx1),
                                          static_cast<float *>(jy1), static_cast<float *>(jz1), st, timer_of(c));
     else
         e = launch_deposit<double, double>(push_geom(c, s->n), sort_arrays<double>(s), true, false,
-                                           bin_state(c), static_cast<double *>(ro),
+                                           bin_state(c, s->column), static_cast<double *>(ro),
                                            static_cast<double *>(jx), static_cast<double *>(jy),
                                            static_cast<double *>(jz),
                                            static_cast<const double *>(bg_ro),
@@ -599,8 +605,8 @@ int lc3d_deposit(lc3d_ctx *c, const lc3d_sort *s, void *ro, void *jx, void *jy, 
                                            static_cast<double *>(jy1), static_cast<double *>(jz1),
                                            st, timer_of(c));
     if (e != hipSuccess) return hip_fail(c, e, "deposit");
-    c->order_x_init = s->x_init;
-    c->order_n = s->n;
+    c->order_x_init[s->column] = s->x_init;
+    c->order_n[s->column] = s->n;
     return LC3D_OK;
 }
 
@@ -719,6 +725,37 @@ int lc3d_energy_flux(lc3d_ctx *c, const lc3d_fields6 *f, int64_t n, const void *
                                        static_cast<const double *>(py), pz, c->diag, DIAG_BLOCKS,
                                        psi, st);
     if (e != hipSuccess) return hip_fail(c, e, "energy_flux");
+    return LC3D_OK;
+}
+
+int lc3d_probe_sample(lc3d_ctx *c, const lc3d_probe_sources *src, int32_t n_points,
+                      const int32_t *ix, const int32_t *iy, double *out, void *stream)
+{
+    if (!c || !src || src->n < 0 || src->n > LC3D_PROBE_MAX_SOURCES || n_points < 0 ||
+        (n_points > 0 && src->n > 0 && (!ix || !iy || !out)))
+        return fail(c, LC3D_EINVAL, "probe_sample: bad arguments");
+    ProbeSources ps{};
+    ps.n = src->n;
+    for (int k = 0; k < src->n; ++k) {
+        if (!src->src[k]) return fail(c, LC3D_EINVAL, "probe_sample: null source");
+        ps.src[k] = src->src[k];
+        ps.is_f32[k] = src->is_f32[k] != 0;
+    }
+    hipError_t e = launch_probe_sample(ps, c->N, n_points, ix, iy, out,
+                                       static_cast<hipStream_t>(stream));
+    if (e != hipSuccess) return hip_fail(c, e, "probe_sample");
+    return LC3D_OK;
+}
+
+int lc3d_abs_max(lc3d_ctx *c, const double *a, int64_t nr, int64_t nc, int64_t r0, int64_t r1,
+                 int64_t c0, int64_t c1, double *out, void *stream)
+{
+    if (!c || !out || nr < 0 || nc < 0 || r0 < 0 || c0 < 0 || r1 > nr || c1 > nc || r0 > r1 ||
+        c0 > c1 || (r1 > r0 && c1 > c0 && !a))
+        return fail(c, LC3D_EINVAL, "abs_max: bad arguments");
+    hipError_t e = launch_absmax(a, nc, r0, r1, c0, c1, c->diag, 2 * DIAG_BLOCKS, out,
+                                 static_cast<hipStream_t>(stream));
+    if (e != hipSuccess) return hip_fail(c, e, "abs_max");
     return LC3D_OK;
 }
 

@@ -208,6 +208,22 @@ hipError_t launch_energy_flux(int N, double h, const Fields6<double> &f, int64_t
                               const TP *px, const TP *py, const double *pz, double *part,
                               int max_blocks, double *psi, hipStream_t s);
 
+// diagnostics/FXi/Strategy3D.py:28-95: each of the ps.n (N, N) sources at
+// the n_points probe cells (ix[p], iy[p]) into out[s * n_points + p], float64.
+constexpr int PROBE_MAX_SOURCES = 16;
+struct ProbeSources {
+    int n;
+    const void *src[PROBE_MAX_SOURCES];
+    int is_f32[PROBE_MAX_SOURCES];
+};
+hipError_t launch_probe_sample(const ProbeSources &ps, int N, int n_points, const int *ix,
+                               const int *iy, double *out, hipStream_t s);
+
+// max |a| over rows [r0, r1) and columns [c0, c1) of an nc-column row-major
+// float64 array into out[0]; part holds max_blocks partial maxima.
+hipError_t launch_absmax(const double *a, int64_t nc, int64_t r0, int64_t r1, int64_t c0,
+                         int64_t c1, double *part, int max_blocks, double *out, hipStream_t s);
+
 // ---- beam (beam3d) -----------------------------------------------------------
 struct BeamGeom {
     int N;

@@ -157,27 +157,99 @@ def init_plasma(config: Config, current_time=0):
     return fields, particles, currents, const_arrays, xi_plasma_layer
 
 
+STATE_FIELDS = ('Ex', 'Ey', 'Ez', 'Bx', 'By', 'Bz', 'Phi')
+STATE_CURRENTS = ('ro', 'jx', 'jy', 'jz')
+STATE_PARTICLE = ('x_init', 'y_init', 'q', 'm', 'x_offt', 'y_offt', 'px', 'py', 'pz',
+                  'dx_chaotic', 'dy_chaotic', 'dx_chaotic_perp', 'dy_chaotic_perp')
+
+
+def _host64(a):
+    import torch
+    if isinstance(a, torch.Tensor):
+        a = a.detach().cpu().numpy()
+    return np.asarray(a, dtype=np.float64)
+
+
+def _particle_record(arrs) -> np.ndarray:
+    """One sort as a structured (n, n) array with a float64 field per
+    quantity: the per-sort entry ``state[sort][field]`` load_plasma reads
+    (initialization.py:430-445).  Quantities a sort lacks are zeros."""
+    first = _host64(getattr(arrs, 'x_init'))
+    rec = np.zeros(first.shape, dtype=[(k, np.float64) for k in STATE_PARTICLE])
+    for k in STATE_PARTICLE:
+        v = getattr(arrs, k, None)
+        if v is not None:
+            rec[k] = _host64(v)
+    return rec
+
+
+def write_plasmastate(path, particles: dict, fields, currents, xi_plasma_layer,
+                      ro_initial=None) -> None:
+    """Writes a plasma state that load_plasma reads back.
+
+    The file holds both layouts of the reference: every sort as a structured
+    record under its name (what load_plasma reads, initialization.py:417-460)
+    and the electrons' quantities as flat keys beside the fields and
+    currents (what SaveRunState writes, diagnostics_3d.py:666-689), plus
+    ``xi_plasma_layer`` and, when given, ``rho_initial``."""
+    data = {'xi_plasma_layer': np.array([float(xi_plasma_layer)])}
+    for sort, arrs in particles.items():
+        data[sort] = _particle_record(arrs)
+    el = particles.get('electrons')
+    if el is not None:
+        for k in STATE_PARTICLE:
+            data[k] = data['electrons'][k].copy()
+    for k in STATE_FIELDS:
+        data[k] = _host64(getattr(fields, k))
+    for k in STATE_CURRENTS:
+        data[k] = _host64(getattr(currents, k))
+    if ro_initial is not None:
+        data['rho_initial'] = _host64(ro_initial)
+    np.savez_compressed(path, **data)
+
+
+def read_plasmastate(path, sorts) -> tuple:
+    """(fields, particles, currents, rho_initial or None, xi_plasma_layer or
+    None) of a plasma state file as host float64 arrays.  A sort is read from
+    its structured record; a file in SaveRunState's flat layout (no record)
+    gives its flat quantities to the electrons."""
+    with np.load(path) as state:
+        files = set(state.files)
+        fields = {k: np.array(state[k], dtype=np.float64) for k in STATE_FIELDS}
+        currents = {k: np.array(state[k], dtype=np.float64) for k in STATE_CURRENTS}
+        particles = {}
+        for sort in sorts:
+            if sort in files:
+                rec = state[sort]
+                particles[sort] = {k: np.array(rec[k], dtype=np.float64) for k in STATE_PARTICLE}
+            elif sort == 'electrons' and 'x_init' in files:
+                particles[sort] = {k: np.array(state[k], dtype=np.float64)
+                                   for k in STATE_PARTICLE}
+            else:
+                raise KeyError(f'plasma state {path}: no particle record for sort {sort!r}')
+        rho_initial = np.array(state['rho_initial']) if 'rho_initial' in files else None
+        xi = float(np.asarray(state['xi_plasma_layer']).ravel()[0]) \
+            if 'xi_plasma_layer' in files else None
+    return fields, particles, currents, rho_initial, xi
+
+
 def load_plasma(config: Config, path_to_plasmastate: str):
-    """initialization.py:417-460: nested ``state[sort][field]`` records, plus
-    fields, currents, optional rho_initial and xi_plasma_layer."""
+    """initialization.py:417-460: the fields, every sort's particles and the
+    currents of a saved state, with rho_initial and xi_plasma_layer when the
+    file holds them (see write_plasmastate / read_plasmastate)."""
     import torch
     fields, particles, currents, const_arrays, xi_plasma_layer = init_plasma(config)
     dt = runtime.Dtypes(config.precision())
     dev = runtime.device()
-    with np.load(path_to_plasmastate) as state:
-        fields = Arrays.wrap(**{k: torch.as_tensor(state[k]).to(device=dev, dtype=dt.full)
-                                for k in ('Ex', 'Ey', 'Ez', 'Bx', 'By', 'Bz', 'Phi')})
-        for sort in const_arrays.sorts:
-            rec = state[sort]
-            host = {k: np.asarray(rec[k]) for k in
-                    ('x_init', 'y_init', 'q', 'm', 'x_offt', 'y_offt', 'px', 'py', 'pz',
-                     'dx_chaotic', 'dy_chaotic', 'dx_chaotic_perp', 'dy_chaotic_perp')}
-            particles[sort] = particles_to_device(config, host)
-        currents = Arrays.wrap(**{k: torch.as_tensor(state[k]).to(device=dev, dtype=dt.current)
-                                  for k in ('ro', 'jx', 'jy', 'jz')})
-        if 'rho_initial' in state.files:
-            const_arrays.ro_initial = torch.as_tensor(state['rho_initial']).to(
-                device=dev, dtype=dt.current)
-        if 'xi_plasma_layer' in state.files:
-            xi_plasma_layer = float(state['xi_plasma_layer'])
+    f, parts, cur, rho_initial, xi = read_plasmastate(path_to_plasmastate, const_arrays.sorts)
+    fields = Arrays.wrap(**{k: torch.as_tensor(f[k]).to(device=dev, dtype=dt.full)
+                            for k in STATE_FIELDS})
+    for sort, host in parts.items():
+        particles[sort] = particles_to_device(config, host)
+    currents = Arrays.wrap(**{k: torch.as_tensor(cur[k]).to(device=dev, dtype=dt.current)
+                              for k in STATE_CURRENTS})
+    if rho_initial is not None:
+        const_arrays.ro_initial = torch.as_tensor(rho_initial).to(device=dev, dtype=dt.current)
+    if xi is not None:
+        xi_plasma_layer = xi
     return fields, particles, currents, const_arrays, xi_plasma_layer

@@ -17,6 +17,7 @@ It restates, in numpy plus the C kernels of ``oracle/kernels.c``:
 * the beam layer bookkeeping    -- beam3d/beam_calculator.py:60-152,
                                    beam3d/beam_io.py:7-66, beam3d/data.py:88-169
 * the xi loop                   -- push_solvers/push_solver.py:215-286
+* the rigid beam                -- beam3d/beam_calculator.py:148-181
 * the time loop                 -- simulation.py:244-264, mpi/beam_io.py:81-119
 * the energy flux diagnostic    -- diagnostics/FXi/Strategy3D.py:6-25
 
@@ -144,15 +145,19 @@ def plasma_grid(steps: int, step_size: float, fineness) -> np.ndarray:
     return np.concatenate([left, right])
 
 
-def uniform_plasma(p: Params) -> dict:
-    """Uniform electron lattice with zero momenta, initialization.py:107-135
-    (uniform density: the profile coefficient is 1)."""
+def uniform_plasma(p: Params, density=None) -> dict:
+    """Electron lattice with zero momenta, initialization.py:107-135.  The
+    density profile coefficient (profile.py:111-145) is 1, or
+    ``density(x_init, y_init)`` when given; it scales q and m alike."""
     g = plasma_grid(p.N - 2 * p.plasma_padding_steps, p.h, p.fineness)
     n = len(g)
     y_init = np.ascontiguousarray(np.broadcast_to(g, (n, n)))
     x_init = y_init.T.copy()
     q = np.ones_like(x_init) * -1 / p.fineness ** 2 * p.h ** 2
     m = np.ones_like(x_init) * 1 / p.fineness ** 2 * p.h ** 2
+    if density is not None:
+        coef = np.asarray(density(x_init, y_init), dtype=np.float64)
+        q, m = q * coef, m * coef
     z = np.zeros_like(x_init)
     return dict(x_init=x_init, y_init=y_init, x_offt=z.copy(), y_offt=z.copy(),
                 px=z.copy(), py=z.copy(), pz=z.copy(), q=q, m=m)
@@ -522,10 +527,29 @@ def layer_to_layout(p: Params, source: dict, plasma_layer_idx: int):
     return ({k: v[:n] for k, v in source.items()}, {k: v[n:] for k, v in source.items()})
 
 
+def rigid_beam_density(p: Params, density, plasma_layer_idx: int) -> np.ndarray:
+    """RigidBeamCalculator.layout_beam_layer (beam3d/beam_calculator.py:148-170):
+    the user's charge density ``density(xp, xi, x_grid, y_grid)`` at
+    xi = -plasma_layer_idx * xi_step on ``meshgrid(grid, grid)`` -- numpy's
+    default 'xy' indexing, so element [i, j] is taken at x = grid[j],
+    y = grid[i] (transposed against the fields' [ix, iy])."""
+    grid = (np.arange(p.N) - p.N // 2) * p.h
+    x_grid, y_grid = np.meshgrid(grid, grid)
+    xi = -plasma_layer_idx * p.xi_step
+    rho = np.asarray(density(np, xi, x_grid, y_grid), dtype=np.float64)
+    return np.ascontiguousarray(np.broadcast_to(rho, (p.N, p.N)))
+
+
 def step_dt(p: Params, fields, particles, currents, const: Const, xi_plasma_layer_start,
-            source: dict, on_step=None):
+            source: dict, on_step=None, rigid=None, n_steps: int | None = None):
     """PusherAndSolver3D.step_dt (push_solver.py:215-286).  Returns
-    (particles, fields, currents, drained_beam, on_axis_Ez list)."""
+    (particles, fields, currents, drained_beam, on_axis_Ez list).
+
+    With ``rigid`` (a density callable, see rigid_beam_density) the beam is
+    the rigid one of beam_calculator.py:148-181: no beam particles move, and
+    the previous layer's density is the last full one.  ``n_steps`` stops
+    the loop after that many xi steps (for checks of the first steps of a
+    long window)."""
     beam_calc = BeamCalc(p)
     beam_calc.start_time_step()
     layer_to_move = empty_beam()
@@ -534,23 +558,38 @@ def step_dt(p: Params, fields, particles, currents, const: Const, xi_plasma_laye
     drain = empty_beam()
     ez_axis = []
     start = round(-xi_plasma_layer_start / p.xi_step) + 1
-    for xi_i in range(start, p.xi_steps + 1):
-        layer, source = layer_to_layout(p, source, xi_i)
-        ro_beam_full = beam_calc.layout_beam_layer(layer, xi_i)
+    stop = p.xi_steps + 1 if n_steps is None else min(p.xi_steps + 1, start + n_steps)
+    for xi_i in range(start, stop):
+        if rigid is not None:
+            ro_beam_full = rigid_beam_density(p, rigid, xi_i)
+        else:
+            layer, source = layer_to_layout(p, source, xi_i)
+            ro_beam_full = beam_calc.layout_beam_layer(layer, xi_i)
         prev_fields = {k: v.copy() for k, v in fields.items()}
         particles, fields, currents = step_dxi(p, particles, fields, currents, const,
                                                ro_beam_full, ro_beam_prev)
-        lost, moved, fell = beam_calc.move_beam_layer(layer_to_move, fell_size, xi_i,
-                                                      prev_fields, fields)
-        layer_to_move = beam_append(layer, fell)
-        fell_size = fell['id'].size
+        if rigid is None:
+            lost, moved, fell = beam_calc.move_beam_layer(layer_to_move, fell_size, xi_i,
+                                                          prev_fields, fields)
+            layer_to_move = beam_append(layer, fell)
+            fell_size = fell['id'].size
+            if moved['id'].size > 0:
+                drain = beam_append(drain, moved)
         ro_beam_prev = ro_beam_full.copy()
-        if moved['id'].size > 0:
-            drain = beam_append(drain, moved)
         ez_axis.append(fields['Ez'][p.N // 2, p.N // 2])
         if on_step is not None:
             on_step(xi_i, particles, fields, currents, ro_beam_full)
     return particles, fields, currents, drain, np.array(ez_axis)
+
+
+def simulate_rigid(p: Params, density, electrons=None, n_steps: int | None = None):
+    """Simulation.step with a rigid beam (simulation.py:194-198, 211-221):
+    one time step of the xi loop with the density callable.  Returns
+    (particles, fields, currents, ez_axis)."""
+    fields, particles, currents, const, xi0 = init_plasma(p, electrons)
+    out = step_dt(p, fields, particles, currents, const, xi0, empty_beam(), rigid=density,
+                  n_steps=n_steps)
+    return out[0], out[1], out[2], out[4]
 
 
 def simulate(p: Params, beam_records: np.ndarray, n_time_steps: int, electrons=None):

@@ -131,3 +131,9 @@ inline hipError_t hipEventRecord(hipEvent_t e, hipStream_t) {
 }
 inline hipError_t hipEventSynchronize(hipEvent_t) { return 0; }
 inline hipError_t hipEventElapsedTime(float *ms, hipEvent_t a, hipEvent_t b) { *ms = (float)(1e3 * (*b - *a)); return 0; }
+
+// Device symbols: the emulation's "device" variables are ordinary host
+// objects, so a symbol's device address is its own address.
+#define HIP_SYMBOL(x) (&(x))
+#define hipErrorInvalidDevice 101
+inline hipError_t hipGetSymbolAddress(void **p, const void *sym) { *p = const_cast<void *>(sym); return 0; }

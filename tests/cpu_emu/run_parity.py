@@ -83,6 +83,20 @@ def _empty_unpinned(*a, **k):
 
 
 torch.empty = _empty_unpinned
+
+
+def _host_factory(fn):
+    def make(*a, **k):
+        dev = k.get('device')
+        if dev == 'cuda' or (isinstance(dev, torch.device) and dev.type == 'cuda'):
+            k['device'] = 'cpu'
+        return fn(*a, **k)
+    return make
+
+
+for _name in ('tensor', 'full', 'zeros', 'ones', 'as_tensor', 'arange'):
+    setattr(torch, _name, _host_factory(getattr(torch, _name)))
+torch.empty = _host_factory(torch.empty)
 _to = torch.Tensor.to
 
 

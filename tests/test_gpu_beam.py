@@ -1,0 +1,129 @@
+"""The beam layer move on the GPU against the oracle, on branches the
+workload beams (gamma ~ 1000, well inside the window) never take:
+
+* substepping: slow particles (gamma of a few) with a high substepping
+  energy get their time step halved several times
+  (beam3d/beam_calculator.py:18-34, 97-114) and run 4 or 8 substeps of the
+  explicit-midpoint loop (beam3d/move.py:31-127);
+* loss: particles that reach the lost radius max(0.9 R, R - 1) at the half
+  or the full substep are flagged lost and their id negated
+  (beam3d/move.py:56-65, 113-118);
+* falling: particles that slip behind the layer at the half step, and the
+  CPU scheme's trailing test on x (beam3d/move.py:123), which flags every
+  particle with x < xi_{k+1}.
+
+The lost, moved and fell groups must hold the same ids, and every particle
+the same state, as the oracle's (float64, 1e-12 relative)."""
+import numpy as np
+import pytest
+
+from oracle import lcode_oracle as O
+
+pytestmark = pytest.mark.gpu
+torch = pytest.importorskip('torch')
+
+N, H, XI_STEP, TIME_STEP, E_SUB = 65, 0.1, 0.05, 0.01, 20.0
+LAYER = 4          # plasma layer index of the move: beam layer LAYER - 1
+
+
+def _config():
+    from lcodepy_amd.config import Config
+    cfg = Config({'geometry': '3d', 'processing-unit-type': 'mi355x', 'transverse-step': H,
+                  'xi-step': XI_STEP, 'window-length': 1.0, 'time-step': TIME_STEP,
+                  'beam-substepping-energy': E_SUB, 'ion-model': 'background',
+                  'field-solver-subtraction-coefficient': 1.0})
+    cfg.set('window-width-steps', N)
+    cfg.set('window-width', N * H)
+    cfg.set('plasma-fineness', 2)
+    cfg.set('plasma-particles-per-cell', 4)
+    return cfg
+
+
+def _params():
+    return O.Params(N=N, h=H, xi_step=XI_STEP, subtraction_coeff=1.0, time_step=TIME_STEP,
+                    substepping_energy=E_SUB)
+
+
+def _smooth(rng, amp):
+    x = np.linspace(0, 1, N)
+    f = np.zeros((N, N))
+    for _ in range(3):
+        a, b, ph = rng.uniform(1, 4), rng.uniform(1, 4), rng.uniform(0, 6)
+        f += rng.normal() * np.sin(a * np.pi * x + ph)[:, None] * np.cos(b * np.pi * x - ph)[None, :]
+    return amp * f
+
+
+def _layer(rng):
+    p = _params()
+    xi_k, xi_k1 = -(LAYER - 1) * XI_STEP, -LAYER * XI_STEP
+    n_fast, n_slow, n_edge, n_back = 300, 120, 60, 40
+    n = n_fast + n_slow + n_edge + n_back
+    b = {k: np.zeros(n) for k in ('xi', 'x', 'y', 'px', 'py', 'pz', 'q_m', 'q_norm', 'dt')}
+    b['id'] = np.arange(1, n + 1, dtype=np.int64)
+    b['remaining_steps'] = np.zeros(n, dtype=np.int64)
+    b['q_m'][:] = -1.0
+    b['q_norm'][:] = -1e-4
+    b['xi'] = rng.uniform(xi_k1 + 0.01, xi_k - 0.001, n)
+    r = rng.uniform(0, 1.5, n)
+    phi = rng.uniform(0, 2 * np.pi, n)
+    b['x'], b['y'] = r * np.cos(phi), r * np.sin(phi)
+    b['pz'] = rng.normal(1000.0, 10.0, n)
+    b['px'] = rng.normal(0, 1.0, n)
+    b['py'] = rng.normal(0, 1.0, n)
+    s = slice(n_fast, n_fast + n_slow)                 # slow: substeps
+    b['pz'][s] = rng.uniform(0.2, 3.0, n_slow)
+    b['px'][s] = rng.normal(0, 0.1, n_slow)
+    b['py'][s] = rng.normal(0, 0.1, n_slow)
+    e = slice(n_fast + n_slow, n_fast + n_slow + n_edge)   # at the lost radius, outward
+    L = p.lost_radius
+    re = L - rng.uniform(0.0005, 0.004, n_edge)
+    b['x'][e], b['y'][e] = re * np.cos(phi[e]), re * np.sin(phi[e])
+    b['px'][e], b['py'][e] = 600.0 * np.cos(phi[e]), 600.0 * np.sin(phi[e])
+    k = slice(n_fast + n_slow + n_edge, n)              # at the back of the layer, slow
+    b['xi'][k] = xi_k1 + rng.uniform(1e-5, 5e-4, n_back)
+    b['pz'][k] = rng.uniform(0.5, 2.0, n_back)
+    return b
+
+
+def test_beam_move_substeps_losses_and_falls_match_oracle():
+    from lcodepy_amd.beam3d.beam_calculator import BeamCalculator
+    from lcodepy_amd.beam3d.data import FLOAT_KEYS, INT_KEYS, BeamParticles
+    from lcodepy_amd.plasma3d.data import Arrays
+    rng = np.random.default_rng(3)
+    b = _layer(rng)
+    p = _params()
+    fk = {k: _smooth(rng, 0.05) for k in ('Ex', 'Ey', 'Ez', 'Bx', 'By', 'Bz')}
+    fk1 = {k: _smooth(rng, 0.05) for k in ('Ex', 'Ey', 'Ez', 'Bx', 'By', 'Bz')}
+    # Fields are gathered only at half-step positions inside the lost radius,
+    # whose 5 x 5 stencils stay on the grid (|x| < (N // 2 - 2) h).
+    assert np.max(np.hypot(b['x'], b['y'])) < p.lost_radius < (N // 2 - 2) * H
+
+    layer_o = {k: v.copy() for k, v in b.items()}
+    lost_o, moved_o, fell_o = O.BeamCalc(p).move_beam_layer(layer_o, 0, LAYER, fk, fk1)
+    mult = layer_o['dt'] / TIME_STEP
+    print(f'beam move: {lost_o["id"].size} lost, {moved_o["id"].size} moved, '
+          f'{fell_o["id"].size} fell; substep multipliers {sorted(set(mult.round(6)))}')
+    assert lost_o['id'].size >= 10 and np.all(lost_o['id'] < 0)
+    assert np.sum(mult <= 0.25) >= 20                 # 4 or more substeps
+    assert np.sum((mult <= 0.25) & (layer_o['remaining_steps'] == 0)
+                  & np.isin(layer_o['id'], moved_o['id'])) >= 10
+
+    print('beam move: gpu', flush=True)
+    cfg = _config()
+    f = torch.tensor(np.stack([b[k] for k in FLOAT_KEYS]), dtype=torch.float64, device='cuda')
+    i = torch.tensor(np.stack([b[k] for k in INT_KEYS]), dtype=torch.int64, device='cuda')
+    layer_g = BeamParticles(f=f, i=i)
+    wrap = lambda d: Arrays.wrap(**{k: torch.tensor(v, device='cuda') for k, v in d.items()})
+    calc = BeamCalculator(cfg)
+    lost_g, moved_g, fell_g = calc.move_beam_layer(layer_g, 0, LAYER, wrap(fk), wrap(fk1))
+    torch.cuda.synchronize()
+    calc.ctx.raise_on_errors()
+    for name, g, o in (('lost', lost_g, lost_o), ('moved', moved_g, moved_o),
+                       ('fell', fell_g, fell_o)):
+        ids = g.id.cpu().numpy()
+        assert np.array_equal(ids, o['id']), name          # same particles, same order
+        for k in ('x', 'y', 'xi', 'px', 'py', 'pz', 'dt'):
+            got = getattr(g, k).cpu().numpy()
+            scale = max(np.max(np.abs(o[k])), 1e-300) if o[k].size else 1.0
+            assert np.max(np.abs(got - o[k]), initial=0.0) <= 1e-12 * scale, (name, k)
+        assert np.array_equal(g.remaining_steps.cpu().numpy(), o['remaining_steps']), name

@@ -356,3 +356,136 @@ def test_energy_flux_single_particle():
     e = {'m': np.array([3.0, 3.0]), 'px': np.array([0.0, 2.0]), 'py': np.array([0.0, 2.0]),
          'pz': np.array([4.0, 2 * np.sqrt(2.0)])}
     assert np.isclose(O.energy_flux(0.1, e, _zero_fields(5)), 4.0, rtol=1e-15)
+
+
+# -- lattice branches (initialization.py:72-104) -----------------------------------
+@pytest.mark.parametrize('steps,fineness', [(99, 1), (739, 3), (27, 5), (99, 0.5), (40, 0.25)])
+def test_plasma_grid_odd_and_fractional_fineness(steps, fineness):
+    """Odd and fractional fineness put a particle on the axis: the lattice
+    has spacing h / fineness, contains 0 exactly, is symmetric, stays inside
+    the plasma width steps * h and has 2 ceil(floor(steps / 2) fineness) - 1
+    points (one fewer than the even-fineness lattice, whose points straddle
+    the axis).  A fraction 0 < f < 1 has f % 2 = f != 0, so it takes the odd
+    branch too: the reference's separate fractional branch
+    (initialization.py:95-97) is never reached."""
+    h = 0.1
+    g = O.plasma_grid(steps, h, fineness)
+    k = math.ceil(steps // 2 * fineness)
+    assert len(g) == 2 * k - 1
+    assert np.any(g == 0.0)
+    assert np.array_equal(g, -g[::-1])
+    assert np.allclose(np.diff(g), h / fineness, rtol=0, atol=1e-12)
+    assert np.max(np.abs(g)) < steps * h / 2
+
+
+@pytest.mark.parametrize('steps,fineness', [(99, 2), (483, 2), (30, 4)])
+def test_plasma_grid_even_fineness_straddles_the_axis(steps, fineness):
+    h = 0.05
+    g = O.plasma_grid(steps, h, fineness)
+    assert len(g) == 2 * (steps // 2 * fineness)
+    assert not np.any(g == 0.0)
+    assert g[len(g) // 2] == pytest.approx(h / fineness / 2, abs=0)
+    assert np.array_equal(g, -g[::-1])
+
+
+def test_lattice_sizes_of_the_workloads():
+    """97^2 electrons at N = 129, 1 ppc; 964^2 at N = 513, 4 ppc; 2213^2 per
+    sort at N = 769, 9 ppc; 1988^2 at N = 1025, 4 ppc (plasma width N - 30 cells)."""
+    for N, fin, n in [(129, 1, 97), (513, 2, 964), (769, 3, 2213), (1025, 2, 1988)]:
+        assert len(O.plasma_grid(N - 30, 0.05, fin)) == n
+
+
+def test_density_profile_scales_charge_and_mass():
+    p = O.Params(N=65, h=0.1, xi_step=0.05, subtraction_coeff=1.0)
+    shape = lambda x, y: np.minimum(1 + 0.1 * (x ** 2 + y ** 2), 2.0)
+    a, u = O.uniform_plasma(p, density=shape), O.uniform_plasma(p)
+    coef = np.minimum(1 + 0.1 * (u['x_init'] ** 2 + u['y_init'] ** 2), 2.0)
+    assert np.array_equal(a['q'], u['q'] * coef) and np.array_equal(a['m'], u['m'] * coef)
+    assert a['q'][0, 0] == pytest.approx(-p.h ** 2 / 4 * coef[0, 0], rel=1e-15)
+
+
+# -- mobile ions (initialization.py:395-407, rhoj.py:11-20) ---------------------------
+def test_mobile_ions_cancel_the_electron_charge_exactly():
+    """The ion sort is the electron lattice with q -> -q, so at t = 0 the
+    two columns of every current are exact negatives (the deposit sums the
+    same terms with the sign flipped): net rho = 0 bit for bit, j = 0."""
+    p = O.Params(N=65, h=0.1, xi_step=0.05, subtraction_coeff=1.0, ion_model='mobile',
+                 ion_mass=1836.0)
+    fields, particles, currents, const, _ = O.init_plasma(p)
+    assert const.sorts == {'electrons': 0, 'ions': 1}
+    ions, el = particles['ions'], particles['electrons']
+    assert np.array_equal(ions['q'], -el['q']) and np.array_equal(ions['m'], 1836.0 * el['m'])
+    cur = O.compute_rhoj(p, particles, const)
+    assert np.any(cur['ro'][0] != 0)
+    assert np.array_equal(cur['ro'][0] + cur['ro'][1], np.zeros((p.N, p.N)))
+    for k in ('jx', 'jy', 'jz'):
+        assert not np.any(cur[k])
+
+
+@pytest.mark.parametrize('ion_mass', [100.0, 1836.0])
+def test_mobile_ion_in_uniform_ez(ion_mass):
+    """An ion of the mobile sort at rest in a uniform Ez: the two-pass
+    predictor gives dp_z = q Ez dxi / (1 - v_z) with v_z taken at the
+    half-step momentum q Ez dxi / 2 and gamma m = sqrt(m^2 + p^2), where
+    q = +h^2 / fineness^2 and m = ion_mass h^2 / fineness^2."""
+    p = O.Params(N=33, h=0.1, xi_step=0.05, subtraction_coeff=1.0, ion_model='mobile',
+                 ion_mass=ion_mass)
+    _, particles, _, const, _ = O.init_plasma(p)
+    ions = {k: np.ascontiguousarray(v[:1, :1]) for k, v in particles['ions'].items()}
+    q, m = p.h ** 2 / 4, ion_mass * p.h ** 2 / 4
+    assert ions['q'][0, 0] == pytest.approx(q, rel=1e-15)
+    assert ions['m'][0, 0] == pytest.approx(m, rel=1e-15)
+    f = O.zero_fields(33)
+    f['Ez'][:] = 40.0
+    full = O.move_wo_fields(p, {'ions': ions})
+    out = O.move_smart(p, f, {'ions': ions}, full)['ions']
+    ph = q * 40.0 * 0.05 / 2
+    vz = ph / math.sqrt(m * m + ph * ph)
+    assert out['pz'][0, 0] == pytest.approx(q * 40.0 * 0.05 / (1 - vz), rel=1e-14)
+    assert out['px'][0, 0] == 0 and out['x_offt'][0, 0] == 0
+
+
+# -- rigid beam (beam3d/beam_calculator.py:148-181) ------------------------------------
+def test_rigid_density_is_the_callable_on_the_xy_meshgrid():
+    """Element [i, j] of the density is the callable at x = grid[j],
+    y = grid[i] (meshgrid's 'xy' indexing), at xi = -layer * xi_step: an
+    asymmetric density tells the two axes apart, and each point is
+    evaluated on its own."""
+    p = O.Params(N=17, h=0.25, xi_step=0.05, subtraction_coeff=1.0)
+    f = lambda xp, xi, x, y: (x + 2.0) * xp.exp(-(y - 0.5) ** 2) * (1.0 + xi)
+    rho = O.rigid_beam_density(p, f, 7)
+    grid = [(k - 8) * 0.25 for k in range(17)]
+    for i in range(17):
+        for j in range(17):
+            want = (grid[j] + 2.0) * math.exp(-(grid[i] - 0.5) ** 2) * (1.0 - 7 * 0.05)
+            assert rho[i, j] == pytest.approx(want, rel=1e-14, abs=1e-300)
+    assert not np.allclose(rho, rho.T)
+
+
+def test_rigid_xi_loop_feeds_each_step_its_layer_and_the_previous_one(monkeypatch):
+    """The rigid loop hands step_dxi the density of layer i as the full
+    density and that of layer i - 1 as the previous one (zero before the
+    first layer), for layers 0..xi_steps; no beam particle is moved."""
+    p = O.Params(N=17, h=0.25, xi_step=0.1, subtraction_coeff=1.0, window_length=0.5,
+                 plasma_padding_steps=3, bound_padding_steps=3)
+    seen = []
+    real = O.step_dxi
+
+    def spy(pp, parts, fields, cur, const, full, prev, trace=None):
+        seen.append((full.copy(), prev.copy()))
+        return real(pp, parts, fields, cur, const, full, prev)
+
+    monkeypatch.setattr(O, 'step_dxi', spy)
+    f = lambda xp, xi, x, y: -0.1 * xp.exp(-(x ** 2 + 2 * y ** 2)) * (xi - 1.0)
+    _, _, _, ez = O.simulate_rigid(p, f)
+    assert len(seen) == 6 and len(ez) == 6
+    grid = (np.arange(17) - 8) * 0.25
+    for i, (full, prev) in enumerate(seen):
+        xi = -i * 0.1
+        want = np.array([[-0.1 * math.exp(-(grid[j] ** 2 + 2 * grid[r] ** 2)) * (xi - 1.0)
+                          for j in range(17)] for r in range(17)])
+        assert np.allclose(full, want, rtol=1e-14, atol=0)
+        if i == 0:
+            assert not np.any(prev)
+        else:
+            assert np.array_equal(prev, seen[i - 1][0])
