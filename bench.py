@@ -41,11 +41,12 @@ TRAFFIC_FILE = os.path.join(ROOT, 'profiles', 'xi_step_513_4ppc_hbm_traffic.md')
 LAUNCHES_PER_STEP = {'push': 3, 'deposit_gather': 3, 'solve_gemm': 8}
 
 
-def workload_config(precision: str) -> dict:
+def workload_config(precision: str, field_solver: str = 'matrix') -> dict:
     return {'geometry': '3d', 'processing-unit-type': 'mi355x', 'precision': precision,
             'window-width': 12.8, 'transverse-step': 0.05,
             'window-length': 20.0, 'xi-step': 0.02, 'time-limit': 1.0, 'time-step': 1.0,
-            'ion-model': 'background', 'plasma-particles-per-cell': 4}
+            'ion-model': 'background', 'plasma-particles-per-cell': 4,
+            'field-solver': field_solver}
 
 
 def driver_beam(n_particles: int, xi_step: float, seed: int = 1) -> np.ndarray:
@@ -96,12 +97,12 @@ def thermalize(particles: dict, m: float, seed: int = 2, sigma: float = 1e-3) ->
 class Run:
     """One precision form of the workload, stepped one xi step at a time."""
 
-    def __init__(self, precision: str, n_beam: int):
+    def __init__(self, precision: str, n_beam: int, field_solver: str = 'matrix'):
         from lcodepy_amd.beam3d.beam_io import BeamDrain, BeamSource
         from lcodepy_amd.config import Config
         from lcodepy_amd.plasma3d.initialization import init_plasma
         from lcodepy_amd.push_solver import PusherAndSolver3D
-        cfg = Config(workload_config(precision))
+        cfg = Config(workload_config(precision, field_solver))
         cfg._adjust_config_values_3d()
         self.cfg = cfg
         self.N = cfg.getint('window-width-steps')
@@ -238,6 +239,7 @@ def main() -> int:
     ap.add_argument('--cpu-sample-steps', type=int, default=20)
     ap.add_argument('--no-cpu-baseline', action='store_true')
     ap.add_argument('--fp64-only', action='store_true')
+    ap.add_argument('--field-solver', default='matrix', choices=('matrix', 'rfft'))
     ap.add_argument('--dump-ez', default=None,
                     help='write the on-axis Ez of every timed xi step of each form (npz)')
     ap.add_argument('--copy-calibration-mib', type=int, default=256,
@@ -270,7 +272,7 @@ def main() -> int:
     forms = ['fp64'] if args.fp64_only else ['fp32-bulk', 'fp64']
     for form in forms:
         print(f'[bench] {form}: building the workload', file=sys.stderr, flush=True)
-        run = Run(form, args.beam_particles)
+        run = Run(form, args.beam_particles, args.field_solver)
         elapsed = time_steps(run, args.warmup, args.steps)
         ez = run.ez()
         kt = live_kernel_times(run, timing_steps)

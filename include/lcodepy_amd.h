@@ -57,6 +57,7 @@ extern "C" {
 #define LC3D_SOLVER_MATRIX 0  /* DST-I/DCT-I as products with N x N transform matrices */
 #define LC3D_SOLVER_FFT 1     /* DST-I/DCT-I as LDS FFTs; only in builds that define
                                  LC3D_FFT_SOLVER (the GPU library does not) */
+#define LC3D_SOLVER_RFFT 2    /* DST-I/DCT-I as paired complex LDS FFTs of length 2(N-1) */
 
 #define LC3D_OK 0
 #define LC3D_EINVAL 1    /* bad argument (null pointer, size beyond the context) */
@@ -82,14 +83,22 @@ typedef struct lc3d_params {
     double lost_radius;          /* max(0.9 R, R - 1), R = h N / 2, beam3d/move.py:285 */
     int32_t precision;           /* LC3D_FP64 or LC3D_FP32_BULK */
     int64_t max_particles;       /* largest plasma sort the context will deposit */
-    /* radix plan of the length-(N-1) complex FFT; 0 stages = library default */
+    /* radix plan of the FFT: of length N-1 for LC3D_SOLVER_FFT, of length
+     * 2(N-1) for LC3D_SOLVER_RFFT (radices 2 3 4 5 7 8 11 13 16);
+     * 0 stages = library default */
     int32_t fft_nstages;
     int32_t fft_radix[12];
     /* host array of the N eigenvalues 4/h^2 sin^2(k pi / (2(N-1))) used by
      * the solver matrices (initialization.py:24, 43-44, 63); NULL = computed
      * here with the C library's sin */
     const double *eigenvalues;
-    int32_t field_solver;        /* LC3D_SOLVER_MATRIX or LC3D_SOLVER_FFT */
+    int32_t field_solver;        /* LC3D_SOLVER_MATRIX, LC3D_SOLVER_FFT or LC3D_SOLVER_RFFT */
+    /* LC3D_SOLVER_RFFT launch shape: LDS of a block in complex values, two
+     * buffers of half of it (float32 solve: 2048, 4096 or 8192; float64:
+     * 1024, 2048 or 4096; 0 = the largest) and transforms per block, each
+     * taking 2(N-1) values of a buffer (0 = as many as fit) */
+    int32_t rfft_cap;
+    int32_t rfft_tpb;
 } lc3d_params;
 
 /* Allocates every workspace (cell bins, spectral scratch, transform

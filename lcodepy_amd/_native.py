@@ -17,7 +17,7 @@ LIB_PATH = os.path.join(_HERE, 'liblcodepy_amd.so')
 
 LC3D_FP64 = 0
 LC3D_FP32_BULK = 1
-SOLVERS = {'matrix': 0, 'fft': 1}
+SOLVERS = {'matrix': 0, 'fft': 1, 'rfft': 2}
 
 ERR_BITS = {
     1: 'plasma particle stencil left the grid (not deposited)',
@@ -45,7 +45,9 @@ class Params(ctypes.Structure):
                 ('fft_nstages', ctypes.c_int32),
                 ('fft_radix', ctypes.c_int32 * 12),
                 ('eigenvalues', ctypes.POINTER(ctypes.c_double)),
-                ('field_solver', ctypes.c_int32)]
+                ('field_solver', ctypes.c_int32),
+                ('rfft_cap', ctypes.c_int32),
+                ('rfft_tpb', ctypes.c_int32)]
 
 
 class Fields6(ctypes.Structure):
@@ -158,7 +160,8 @@ class Context:
     def __init__(self, N: int, h: float, xi_step: float, boundary: float,
                  subtraction_coeff: float, lost_radius: float, precision: str,
                  max_particles: int, eigenvalues: np.ndarray | None = None,
-                 fft_radix: tuple | None = None, field_solver: str = 'matrix'):
+                 fft_radix: tuple | None = None, field_solver: str = 'matrix',
+                 rfft_cap: int = 0, rfft_tpb: int = 0):
         import torch
         if not torch.cuda.is_available():
             raise RuntimeError('lcodepy_amd needs a ROCm GPU (torch.cuda.is_available() is False)')
@@ -179,6 +182,8 @@ class Context:
         if field_solver not in SOLVERS:
             raise ValueError(f'unknown field solver {field_solver!r} (one of {sorted(SOLVERS)})')
         p.field_solver = SOLVERS[field_solver]
+        p.rfft_cap = int(rfft_cap)
+        p.rfft_tpb = int(rfft_tpb)
         if fft_radix:
             p.fft_nstages = len(fft_radix)
             for i, r in enumerate(fft_radix):
@@ -191,8 +196,9 @@ class Context:
         rc = self.lib.lc3d_create(ctypes.byref(p), ctypes.byref(h_))
         if rc != 0:
             raise RuntimeError(f'lc3d_create failed with code {rc} (N={N}, precision={precision}, '
-                               f'field solver {field_solver}; the LDS-FFT solver is not built '
-                               'into the GPU library)')
+                               f'field solver {field_solver}, FFT plan {fft_radix}, rfft shape '
+                               f'{rfft_cap}/{rfft_tpb}; the "fft" solver is not built into the '
+                               'GPU library)')
         self.handle = h_
 
     def __del__(self):

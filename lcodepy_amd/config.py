@@ -198,8 +198,8 @@ class Config:
 
     def field_solver(self) -> str:
         s = self.get('field-solver', 'matrix').lower()
-        if s not in ('matrix', 'fft'):
-            raise ValueError(f"field-solver must be 'matrix' or 'fft', got {s!r}")
+        if s not in ('matrix', 'fft', 'rfft'):
+            raise ValueError(f"field-solver must be 'matrix', 'fft' or 'rfft', got {s!r}")
         return s
 
     def precision(self) -> str:

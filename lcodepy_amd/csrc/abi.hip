@@ -41,6 +41,8 @@ struct lc3d_ctx {
     double *rowsum = nullptr;
     double *diag = nullptr;   // energy-flux partial sums, 2 DIAG_BLOCKS
     void *tw = nullptr;                                      // FFT twiddles
+    uint64_t rf_stages = 0;                                  // RFFT radix plan (packed)
+    int rf_cap = 0, rf_tpb = 0;                              // RFFT launch shape
     void *tdst = nullptr, *tdct = nullptr, *tdct_t = nullptr;   // transform matrices
     double *lam = nullptr;
     KernelTimer timer;
@@ -104,6 +106,44 @@ bool valid_plan(const FftPlan &p)
     return prod == p.M;
 }
 
+// Default radix plan of the length-L FFT of the RFFT solve: radix 16 while
+// it divides, then 8, 4 or 2 for the rest of the power of two, then the odd
+// primes.  Returns the packed plan (radix of stage s in bits 5s..5s+4), or 0
+// when L has another prime factor or needs more stages than fit.
+uint64_t rfft_default_plan(int L)
+{
+    std::vector<int> r;
+    int m = L;
+    while (m % 16 == 0) { r.push_back(16); m /= 16; }
+    if (m % 8 == 0) { r.push_back(8); m /= 8; }
+    if (m % 4 == 0) { r.push_back(4); m /= 4; }
+    if (m % 2 == 0) { r.push_back(2); m /= 2; }
+    for (int q : {3, 5, 7, 11, 13})
+        while (m % q == 0) { r.push_back(q); m /= q; }
+    if (m != 1 || r.size() > (size_t)RFFT_MAX_STAGES) return 0;
+    uint64_t packed = 0;
+    for (size_t s = 0; s < r.size(); ++s) packed |= (uint64_t)r[s] << (5 * s);
+    return packed;
+}
+
+// Packs a caller's plan for the length-L FFT; 0 when it is not one.
+uint64_t rfft_pack_plan(int L, int ns, const int32_t *radix)
+{
+    if (ns < 1 || ns > RFFT_MAX_STAGES) return 0;
+    int64_t prod = 1;
+    uint64_t packed = 0;
+    for (int s = 0; s < ns; ++s) {
+        const int r = radix[s];
+        if (r != 2 && r != 3 && r != 4 && r != 5 && r != 7 && r != 8 && r != 11 && r != 13 &&
+            r != 16)
+            return 0;
+        prod *= r;
+        if (prod > L) return 0;
+        packed |= (uint64_t)r << (5 * s);
+    }
+    return prod == L ? packed : 0;
+}
+
 bool fp32(const lc3d_ctx *c) { return c->prm.precision == LC3D_FP32_BULK; }
 
 PushGeom push_geom(const lc3d_ctx *c, int64_t n)
@@ -162,6 +202,12 @@ SolveGeom solve_geom(const lc3d_ctx *c)
 }
 #endif
 
+RfftGeom rfft_geom(const lc3d_ctx *c)
+{
+    return RfftGeom{c->N, c->prm.grid_step_size, c->prm.xi_step_size, c->prm.subtraction_coeff,
+                    c->rf_stages, c->rf_cap, c->rf_tpb, c->lam, c->tw, c->work, c->rowsum};
+}
+
 MatrixGeom matrix_geom(const lc3d_ctx *c)
 {
     return MatrixGeom{c->N, c->prm.grid_step_size, c->prm.xi_step_size, c->prm.subtraction_coeff,
@@ -182,6 +228,9 @@ hipError_t run_solve(const lc3d_ctx *c, const SolveIn<TG, TPRED> &in,
     if (c->solver == LC3D_SOLVER_FFT)
         return launch_compute_fields<TG, TPRED, TFULL, THALF>(solve_geom(c), in, out, write_full, st);
 #endif
+    if (c->solver == LC3D_SOLVER_RFFT)
+        return launch_compute_fields_rfft<TG, TPRED, TFULL, THALF>(rfft_geom(c), in, out,
+                                                                   write_full, st, timer_of(c));
     return launch_compute_fields_matrix<TG, TPRED, TFULL, THALF>(
         matrix_geom(c), in, out, write_full, st, timer_of(c));
 }
@@ -357,9 +406,23 @@ int lc3d_create(const lc3d_params *p, lc3d_ctx **out)
 #else
     const bool fft = false;
 #endif
-    if (c->solver != LC3D_SOLVER_MATRIX && !fft) {
+    const bool rfft = c->solver == LC3D_SOLVER_RFFT;
+    if (c->solver != LC3D_SOLVER_MATRIX && !fft && !rfft) {
         delete c;
         return LC3D_EINVAL;   // unknown solver, or the FFT solver in a build without it
+    }
+    if (rfft) {
+        const int L = 2 * M;
+        const bool f32s = p->precision == LC3D_FP32_BULK;
+        c->rf_stages = p->fft_nstages > 0 ? rfft_pack_plan(L, p->fft_nstages, p->fft_radix)
+                                          : rfft_default_plan(L);
+        c->rf_cap = p->rfft_cap > 0 ? p->rfft_cap : (f32s ? 8192 : 4096);
+        c->rf_tpb = p->rfft_tpb > 0 ? p->rfft_tpb : c->rf_cap / (2 * L);
+        if (c->rf_stages == 0 || !rfft_cap_supported(f32s, c->rf_cap) || c->rf_tpb < 1 ||
+            2 * (int64_t)c->rf_tpb * L > c->rf_cap) {
+            delete c;
+            return LC3D_EINVAL;
+        }
     }
     if (fft) {
         if (p->fft_nstages > 0) {
@@ -398,8 +461,8 @@ int lc3d_create(const lc3d_params *p, lc3d_ctx **out)
               hipMalloc(&c->rowsum, N * sizeof(double)) == hipSuccess &&
               hipMalloc(&c->diag, 2 * DIAG_BLOCKS * sizeof(double)) == hipSuccess &&
               hipMalloc(&c->lam, N * sizeof(double)) == hipSuccess;
-    if (ok && fft) ok = hipMalloc(&c->tw, 2 * (size_t)M * 2 * tg) == hipSuccess;
-    if (ok && !fft)
+    if (ok && (fft || rfft)) ok = hipMalloc(&c->tw, 2 * (size_t)M * 2 * tg) == hipSuccess;
+    if (ok && !fft && !rfft)
         ok = hipMalloc(&c->work2, 7 * nn * tg) == hipSuccess &&
              hipMalloc(&c->tdst, nn * tg) == hipSuccess &&
              hipMalloc(&c->tdct, nn * tg) == hipSuccess &&
@@ -427,8 +490,8 @@ int lc3d_create(const lc3d_params *p, lc3d_ctx **out)
         return hipMemcpy(dst, v.data(), v.size() * sizeof(double), hipMemcpyHostToDevice);
     };
     hipError_t e = hipSuccess;
-    if (fft) {
-        // Twiddles exp(-i pi t / M), t = 0..2M-1.
+    if (fft || rfft) {
+        // Twiddles exp(-i pi t / M) = exp(-2 pi i t / L), L = 2M, t = 0..L-1.
         std::vector<double> twd(4 * (size_t)M);
         for (int t = 0; t < 2 * M; ++t) {
             twd[2 * t] = std::cos(pi * t / M);

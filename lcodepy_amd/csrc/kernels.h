@@ -200,6 +200,31 @@ hipError_t launch_compute_fields_matrix(const MatrixGeom &g, const SolveIn<TG, T
                                         const SolveOut<TFULL, THALF> &out, bool write_full,
                                         hipStream_t s, KernelTimer *timer = nullptr);
 
+// The same solve with each DST-I / DCT-I computed by a complex FFT of length
+// L = 2(N-1) in LDS, two real transforms per FFT (rfft.hip).
+constexpr int RFFT_MAX_STAGES = 12;
+struct RfftGeom {
+    int N;
+    double h, xi_step, c;   // c: subtraction coefficient
+    uint64_t stages;        // radix of stage s in bits 5s..5s+4 (2 3 4 5 7 8 11 13 16),
+                            // 0 after the last stage; the radices multiply to L
+    int cap;                // LDS of a block in complex values, two buffers of cap / 2
+                            // (an instantiated size)
+    int tpb;                // transforms per block, tpb * L <= cap / 2
+    const double *lam;      // N eigenvalues 4/h^2 sin^2(k pi / 2(N-1))
+    const void *tw;         // L roots exp(-2 pi i t / L), complex TG
+    void *work;             // 7 N*N TG
+    double *rowsum;         // N
+};
+
+// LDS capacities (complex values per block) the FFT kernels are built for.
+bool rfft_cap_supported(bool fp32, int cap);
+
+template <typename TG, typename TPRED, typename TFULL, typename THALF>
+hipError_t launch_compute_fields_rfft(const RfftGeom &g, const SolveIn<TG, TPRED> &in,
+                                      const SolveOut<TFULL, THALF> &out, bool write_full,
+                                      hipStream_t s, KernelTimer *timer = nullptr);
+
 // ---- diagnostics ---------------------------------------------------------------
 // diagnostics/FXi/Strategy3D.py:6-25: total energy flux of a plasma layer into
 // psi[0]; part holds 2 max_blocks float64 partial sums.

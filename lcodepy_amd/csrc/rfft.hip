@@ -1,0 +1,469 @@
+// Field solve of one xi step (plasma3d/fields.py:112-322) with every DST-I /
+// DCT-I computed by a complex FFT of length L = 2(N-1) in LDS, gfx950.
+//
+// scipy.fftpack's DST-I and DCT-I of x_0..x_M (M = N - 1; the DST reads
+// x_1..x_{M-1}), the transforms of fields.py:15-109, are the length-L DFT of
+// x's odd / even period-L extension:
+//     DCT-I(x)_k =  Re DFT(even extension)_k   (that DFT is real),
+//     DST-I(x)_k = -Im DFT(odd extension)_k    (that DFT is imaginary).
+// Two sequences a, b of one kind therefore share one complex FFT of
+// z = ext(a) + i ext(b), Z_k = A_k + i B_k:
+//     DCT: Y^a_k =  Re Z_k,  Y^b_k = Im Z_k;
+//     DST: Y^a_k = -Im Z_k,  Y^b_k = Re Z_k.
+// Each transform is its own inverse up to the factor L, so the solve of
+// field f with right-hand side R is Ty Tx (Lambda .* (Tx Ty R)), Lambda the
+// solver matrix of initialization.py:15-67 divided by L^2 (solver_factor).
+//
+// Kernels (all seven fields in one launch, blockIdx.y = field; rows / columns
+// paired (2p, 2p + 1), the last pair of the odd N holding one):
+//   k_rf_rows_fwd  right-hand sides of two rows (evaluated here from the
+//                  currents), transform along y, spectral rows into work;
+//   k_rf_cols      two spectral columns: transform along x, times Lambda,
+//                  transform back along x, in place in work;
+//   k_rf_rows_inv  two rows back along y; outputs (fields.py:299-314) and the
+//                  Bz row sums;
+//   k_bz_finalize  removes the Bz mean (fields.py:264).
+// The FFT is a Stockham autosort FFT on the tpb transforms of a block at
+// once, each stage from one LDS buffer into the other (one barrier per
+// stage).  The radix sequence is a host plan packed into one integer;
+// twiddles are a host-built float64 table of exp(-2 pi i t / L) in the
+// solve's type.
+#include "solve_common.h"
+
+// Index checks of every LDS and work-array access, compiled only into the
+// host build of these kernels (tests/cpu_emu defines RF_CHECK).
+#ifndef RF_CHECK
+#define RF_CHECK(cond) ((void)0)
+#endif
+
+namespace lc3d {
+
+constexpr int RF_T = 256;   // threads per block
+
+template <typename T> struct Cx {
+    using C = cplx_t<T>;
+    static __device__ __forceinline__ C make(T re, T im) { C r; r.x = re; r.y = im; return r; }
+    static __device__ __forceinline__ C add(C a, C b) { return make(a.x + b.x, a.y + b.y); }
+    static __device__ __forceinline__ C sub(C a, C b) { return make(a.x - b.x, a.y - b.y); }
+    static __device__ __forceinline__ C mul(C a, C b) {
+        return make(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+    }
+    static __device__ __forceinline__ C mul_c(C a, T re, T im) {   // a * (re + i im)
+        return make(a.x * re - a.y * im, a.x * im + a.y * re);
+    }
+    static __device__ __forceinline__ C negi(C a) { return make(a.y, -a.x); }   // a * (-i)
+    static __device__ __forceinline__ C neg(C a) { return make(-a.x, -a.y); }
+};
+
+// Forward 4-point DFT in place.
+template <typename T>
+__device__ __forceinline__ void rf_dft4(cplx_t<T> &a0, cplx_t<T> &a1, cplx_t<T> &a2, cplx_t<T> &a3)
+{
+    using X = Cx<T>;
+    const cplx_t<T> s02 = X::add(a0, a2), d02 = X::sub(a0, a2);
+    const cplx_t<T> s13 = X::add(a1, a3), d13 = X::negi(X::sub(a1, a3));
+    a0 = X::add(s02, s13);
+    a1 = X::add(d02, d13);
+    a2 = X::sub(s02, s13);
+    a3 = X::sub(d02, d13);
+}
+
+// R-point forward DFT y_q = sum_p v_p exp(-2 pi i p q / R) in place.  The
+// general form (odd R) reads exp(-2 pi i m / R) from the twiddle table.
+template <typename T, int R> struct RfDft {
+    static __device__ __forceinline__ void run(cplx_t<T> (&v)[R], const cplx_t<T> *__restrict__ tw,
+                                               int L)
+    {
+        using X = Cx<T>;
+        const int step = L / R;
+        cplx_t<T> y[R];
+#pragma unroll
+        for (int q = 0; q < R; ++q) {
+            cplx_t<T> s = v[0];
+#pragma unroll
+            for (int p = 1; p < R; ++p)
+                s = (q == 0) ? X::add(s, v[p]) : X::add(s, X::mul(v[p], tw[((p * q) % R) * step]));
+            y[q] = s;
+        }
+#pragma unroll
+        for (int q = 0; q < R; ++q) v[q] = y[q];
+    }
+};
+
+template <typename T> struct RfDft<T, 2> {
+    static __device__ __forceinline__ void run(cplx_t<T> (&v)[2], const cplx_t<T> *, int)
+    {
+        const cplx_t<T> a = v[0], b = v[1];
+        v[0] = Cx<T>::add(a, b);
+        v[1] = Cx<T>::sub(a, b);
+    }
+};
+
+template <typename T> struct RfDft<T, 3> {
+    static __device__ __forceinline__ void run(cplx_t<T> (&v)[3], const cplx_t<T> *, int)
+    {
+        using X = Cx<T>;
+        const T h3 = T(0.86602540378443864676);   // sqrt(3) / 2
+        const cplx_t<T> t = X::add(v[1], v[2]);
+        const cplx_t<T> d = X::negi(X::sub(v[1], v[2]));   // -i (v1 - v2)
+        const cplx_t<T> m = X::make(v[0].x - T(0.5) * t.x, v[0].y - T(0.5) * t.y);
+        v[0] = X::add(v[0], t);
+        v[1] = X::make(m.x + h3 * d.x, m.y + h3 * d.y);
+        v[2] = X::make(m.x - h3 * d.x, m.y - h3 * d.y);
+    }
+};
+
+template <typename T> struct RfDft<T, 4> {
+    static __device__ __forceinline__ void run(cplx_t<T> (&v)[4], const cplx_t<T> *, int)
+    {
+        rf_dft4<T>(v[0], v[1], v[2], v[3]);
+    }
+};
+
+// 8 = 2 x 4: DFTs of the even and odd samples, combined with exp(-i pi k / 4).
+template <typename T> struct RfDft<T, 8> {
+    static __device__ __forceinline__ void run(cplx_t<T> (&v)[8], const cplx_t<T> *, int)
+    {
+        using X = Cx<T>;
+        const T r2 = T(0.70710678118654752440);   // sqrt(1/2)
+        cplx_t<T> e0 = v[0], e1 = v[2], e2 = v[4], e3 = v[6];
+        cplx_t<T> o0 = v[1], o1 = v[3], o2 = v[5], o3 = v[7];
+        rf_dft4<T>(e0, e1, e2, e3);
+        rf_dft4<T>(o0, o1, o2, o3);
+        o1 = X::make(r2 * (o1.x + o1.y), r2 * (o1.y - o1.x));    // * exp(-i pi / 4)
+        o2 = X::negi(o2);                                         // * exp(-i pi / 2)
+        o3 = X::make(r2 * (o3.y - o3.x), -r2 * (o3.x + o3.y));   // * exp(-3 i pi / 4)
+        v[0] = X::add(e0, o0);
+        v[4] = X::sub(e0, o0);
+        v[1] = X::add(e1, o1);
+        v[5] = X::sub(e1, o1);
+        v[2] = X::add(e2, o2);
+        v[6] = X::sub(e2, o2);
+        v[3] = X::add(e3, o3);
+        v[7] = X::sub(e3, o3);
+    }
+};
+
+// 16 = 4 x 4: n = 4 n1 + n2, k = k1 + 4 k2.  DFT4 over n1 for each n2, times
+// w^(n2 k1) (w = exp(-2 pi i / 16)), DFT4 over n2 for each k1.
+template <typename T> struct RfDft<T, 16> {
+    static __device__ __forceinline__ void run(cplx_t<T> (&v)[16], const cplx_t<T> *, int)
+    {
+        using X = Cx<T>;
+        const T c8 = T(0.92387953251128675613);   // cos(pi / 8)
+        const T s8 = T(0.38268343236508977173);   // sin(pi / 8)
+        const T r2 = T(0.70710678118654752440);
+#pragma unroll
+        for (int n2 = 0; n2 < 4; ++n2) rf_dft4<T>(v[n2], v[4 + n2], v[8 + n2], v[12 + n2]);
+        // v[4 k1 + n2] now holds the k1-th output of group n2
+        v[4 + 1] = X::mul_c(v[4 + 1], c8, -s8);     // w^1
+        v[4 + 2] = X::mul_c(v[4 + 2], r2, -r2);     // w^2
+        v[4 + 3] = X::mul_c(v[4 + 3], s8, -c8);     // w^3
+        v[8 + 1] = X::mul_c(v[8 + 1], r2, -r2);     // w^2
+        v[8 + 2] = X::negi(v[8 + 2]);               // w^4
+        v[8 + 3] = X::mul_c(v[8 + 3], -r2, -r2);    // w^6
+        v[12 + 1] = X::mul_c(v[12 + 1], s8, -c8);   // w^3
+        v[12 + 2] = X::mul_c(v[12 + 2], -r2, -r2);  // w^6
+        v[12 + 3] = X::mul_c(v[12 + 3], -c8, s8);   // w^9
+        cplx_t<T> y[16];
+#pragma unroll
+        for (int k1 = 0; k1 < 4; ++k1) {
+            cplx_t<T> b0 = v[4 * k1], b1 = v[4 * k1 + 1], b2 = v[4 * k1 + 2], b3 = v[4 * k1 + 3];
+            rf_dft4<T>(b0, b1, b2, b3);
+            y[k1] = b0;
+            y[k1 + 4] = b1;
+            y[k1 + 8] = b2;
+            y[k1 + 12] = b3;
+        }
+#pragma unroll
+        for (int k = 0; k < 16; ++k) v[k] = y[k];
+    }
+};
+
+// One Stockham stage of radix R over the ntr transforms of length L in src
+// (transform t at src[t L ..]) into dst; Ns is the product of the earlier
+// radices.  Butterfly b of a transform reads x[b + r L/R], twiddles input r
+// by exp(-2 pi i r k / (Ns R)), k = b mod Ns, and writes
+// y[(b - k) R + k + r Ns].  Every index stays below L: Ns R divides L.
+template <typename T, int R>
+__device__ __forceinline__ void rf_stage(const cplx_t<T> *src, cplx_t<T> *dst,
+                                         const cplx_t<T> *__restrict__ tw, int L, int ntr, int Ns)
+{
+    using C = cplx_t<T>;
+    using X = Cx<T>;
+    const int LR = L / R, total = ntr * LR, tws = L / (Ns * R);
+    for (int g = threadIdx.x; g < total; g += RF_T) {
+        const int tr = g / LR, b = g - tr * LR, k = b % Ns;
+        const C *in = src + tr * L + b;
+        RF_CHECK(tr < ntr && b + (R - 1) * LR < L);
+        C v[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) v[r] = in[r * LR];
+        if (k > 0) {
+#pragma unroll
+            for (int r = 1; r < R; ++r) v[r] = X::mul(v[r], tw[r * k * tws]);
+        }
+        RfDft<T, R>::run(v, tw, L);
+        C *out = dst + tr * L + (b - k) * R + k;
+        RF_CHECK((b - k) * R + k + (R - 1) * Ns < L && (R - 1) * k * tws < L);
+#pragma unroll
+        for (int r = 0; r < R; ++r) out[r * Ns] = v[r];
+    }
+    __syncthreads();
+}
+
+// Forward FFT of the ntr length-L transforms in a, radices from `stages`,
+// alternating between the LDS buffers a and b; returns the one holding the
+// result.  Every thread of the block runs every stage (each ends with a
+// barrier).
+template <typename T>
+__device__ __forceinline__ cplx_t<T> *rf_fft(cplx_t<T> *a, cplx_t<T> *b,
+                                             const cplx_t<T> *__restrict__ tw, uint64_t stages,
+                                             int L, int ntr)
+{
+    int Ns = 1;
+    for (int s = 0; s < RFFT_MAX_STAGES; ++s) {
+        const int R = (int)((stages >> (5 * s)) & 31);
+        if (R == 0) break;
+        switch (R) {
+        case 2: rf_stage<T, 2>(a, b, tw, L, ntr, Ns); break;
+        case 3: rf_stage<T, 3>(a, b, tw, L, ntr, Ns); break;
+        case 4: rf_stage<T, 4>(a, b, tw, L, ntr, Ns); break;
+        case 5: rf_stage<T, 5>(a, b, tw, L, ntr, Ns); break;
+        case 7: rf_stage<T, 7>(a, b, tw, L, ntr, Ns); break;
+        case 8: rf_stage<T, 8>(a, b, tw, L, ntr, Ns); break;
+        case 11: rf_stage<T, 11>(a, b, tw, L, ntr, Ns); break;
+        case 13: rf_stage<T, 13>(a, b, tw, L, ntr, Ns); break;
+        case 16: rf_stage<T, 16>(a, b, tw, L, ntr, Ns); break;
+        default: break;   // the host accepts only the radices above
+        }
+        cplx_t<T> *t = a;
+        a = b;
+        b = t;
+        Ns *= R;
+    }
+    return a;
+}
+
+// Transforms (row or column pairs) of block bx: min(tpb, pairs - bx tpb).
+__device__ __forceinline__ int rf_block_transforms(int tpb, int pairs, int bx)
+{
+    const int left = pairs - bx * tpb;
+    return left < tpb ? left : tpb;
+}
+
+// ---- rows, forward: right-hand sides and the transform along y ---------------
+// CAP: complex values of the block's LDS, two buffers of CAP / 2.
+template <typename TG, typename TPRED, int CAP>
+__global__ __launch_bounds__(RF_T) void k_rf_rows_fwd(RfftGeom g, RhsEval<TG, TPRED> rhs)
+{
+    using C = cplx_t<TG>;
+    using X = Cx<TG>;
+    __shared__ __align__(16) C lds[CAP];
+    const int f = blockIdx.y, N = g.N, M = N - 1, L = 2 * M, per = M + 1;
+    const int p0 = blockIdx.x * g.tpb;
+    const int ntr = rf_block_transforms(g.tpb, (N + 1) / 2, blockIdx.x);
+    const bool ydst = dst_along_y(f);
+    const C *tw = static_cast<const C *>(g.tw);
+    TG *work = static_cast<TG *>(g.work) + (size_t)f * N * N;
+    RF_CHECK(ntr >= 1 && 2 * ntr * L <= CAP);
+    for (int it = threadIdx.x; it < ntr * per; it += RF_T) {
+        const int tr = it / per, j = it - tr * per;
+        const int ia = 2 * (p0 + tr), ib = ia + 1;
+        RF_CHECK(ia < N && j <= M);
+        const TG a = in_rhs_domain(f, ia, j, N) ? rhs(f, ia, j) : TG(0);
+        const TG b = (ib < N && in_rhs_domain(f, ib, j, N)) ? rhs(f, ib, j) : TG(0);
+        C *t = lds + tr * L;
+        t[j] = X::make(a, b);
+        if (j > 0 && j < M) t[L - j] = ydst ? X::make(-a, -b) : X::make(a, b);
+    }
+    __syncthreads();
+    const C *z = rf_fft<TG>(lds, lds + CAP / 2, tw, g.stages, L, ntr);
+    for (int it = threadIdx.x; it < ntr * per; it += RF_T) {
+        const int tr = it / per, k = it - tr * per;
+        const int ia = 2 * (p0 + tr), ib = ia + 1;
+        const C v = z[tr * L + k];
+        const bool zero = ydst && (k == 0 || k == M);
+        const TG ya = zero ? TG(0) : (ydst ? -v.y : v.x);
+        const TG yb = zero ? TG(0) : (ydst ? v.x : v.y);
+        work[(size_t)ia * N + k] = ya;
+        if (ib < N) work[(size_t)ib * N + k] = yb;
+    }
+}
+
+// ---- columns: along x, times the solver matrix, back along x -----------------
+template <typename TG, int CAP>
+__global__ __launch_bounds__(RF_T) void k_rf_cols(RfftGeom g)
+{
+    using C = cplx_t<TG>;
+    using X = Cx<TG>;
+    __shared__ __align__(16) C lds[CAP];
+    const int f = blockIdx.y, N = g.N, M = N - 1, L = 2 * M, per = M + 1;
+    const int p0 = blockIdx.x * g.tpb;
+    const int ntr = rf_block_transforms(g.tpb, (N + 1) / 2, blockIdx.x);
+    const int j0 = 2 * p0, nc = 2 * ntr;   // columns j0 .. j0 + nc - 1 (the last may be N)
+    const bool xdst = dst_along_x(f);
+    const C *tw = static_cast<const C *>(g.tw);
+    TG *work = static_cast<TG *>(g.work) + (size_t)f * N * N;
+    C *a = lds, *b = lds + CAP / 2;
+    TG *ar = reinterpret_cast<TG *>(a);
+    RF_CHECK(ntr >= 1 && 2 * ntr * L <= CAP && j0 + nc <= N + 1);
+    // Consecutive threads take consecutive columns of one row.
+    for (int it = threadIdx.x; it < per * nc; it += RF_T) {
+        const int i = it / nc, c = it - i * nc;
+        const int j = j0 + c, tr = c >> 1, part = c & 1;
+        TG x = j < N ? work[(size_t)i * N + j] : TG(0);
+        if (xdst && (i == 0 || i == M)) x = TG(0);
+        ar[2 * (tr * L + i) + part] = x;
+        if (i > 0 && i < M) ar[2 * (tr * L + L - i) + part] = xdst ? -x : x;
+    }
+    __syncthreads();
+    C *z = rf_fft<TG>(a, b, tw, g.stages, L, ntr);
+    C *e = z == a ? b : a;
+    // Spectra times the solver matrix (zero at DST edges), written as the
+    // extension the inverse transform along x starts from.
+    const ScaleGeom sg{g.lam, g.c, (double)L * (double)L};
+    for (int it = threadIdx.x; it < ntr * per; it += RF_T) {
+        const int tr = it / per, k = it - tr * per;
+        const int ja = j0 + 2 * tr, jb = ja + 1;
+        const C v = z[tr * L + k];
+        const TG ya = xdst ? -v.y : v.x;
+        const TG yb = xdst ? v.x : v.y;
+        const double fa = solver_factor(f, k, ja, N, sg);
+        const double fb = jb < N ? solver_factor(f, k, jb, N, sg) : 0.0;
+        const C s = X::make((TG)((double)ya * fa), (TG)((double)yb * fb));
+        C *t = e + tr * L;
+        t[k] = s;
+        if (k > 0 && k < M) t[L - k] = xdst ? X::neg(s) : s;
+    }
+    __syncthreads();
+    z = rf_fft<TG>(e, z, tw, g.stages, L, ntr);
+    for (int it = threadIdx.x; it < per * nc; it += RF_T) {
+        const int i = it / nc, c = it - i * nc;
+        const int j = j0 + c, tr = c >> 1, part = c & 1;
+        if (j < N) {
+            const C v = z[tr * L + i];
+            TG x;
+            if (xdst) x = (i == 0 || i == M) ? TG(0) : (part ? v.x : -v.y);
+            else x = part ? v.y : v.x;
+            work[(size_t)i * N + j] = x;
+        }
+    }
+}
+
+// ---- rows, inverse: back along y, outputs and the Bz row sums ----------------
+template <typename TG, typename TFULL, typename THALF, int CAP>
+__global__ __launch_bounds__(RF_T) void k_rf_rows_inv(RfftGeom g, SolveOut<TFULL, THALF> out,
+                                                      bool write_full)
+{
+    using C = cplx_t<TG>;
+    using X = Cx<TG>;
+    __shared__ __align__(16) C lds[CAP];
+    const int f = blockIdx.y, N = g.N, M = N - 1, L = 2 * M, per = M + 1;
+    const int p0 = blockIdx.x * g.tpb;
+    const int ntr = rf_block_transforms(g.tpb, (N + 1) / 2, blockIdx.x);
+    const bool ydst = dst_along_y(f);
+    const C *tw = static_cast<const C *>(g.tw);
+    const TG *work = static_cast<const TG *>(g.work) + (size_t)f * N * N;
+    RF_CHECK(ntr >= 1 && 2 * ntr * L <= CAP);
+    for (int it = threadIdx.x; it < ntr * per; it += RF_T) {
+        const int tr = it / per, k = it - tr * per;
+        const int ia = 2 * (p0 + tr), ib = ia + 1;
+        RF_CHECK(ia < N && k <= M);
+        TG a = work[(size_t)ia * N + k];
+        TG b = ib < N ? work[(size_t)ib * N + k] : TG(0);
+        if (ydst && (k == 0 || k == M)) a = b = TG(0);
+        C *t = lds + tr * L;
+        t[k] = X::make(a, b);
+        if (k > 0 && k < M) t[L - k] = ydst ? X::make(-a, -b) : X::make(a, b);
+    }
+    __syncthreads();
+    const C *z = rf_fft<TG>(lds, lds + CAP / 2, tw, g.stages, L, ntr);
+    for (int it = threadIdx.x; it < ntr * per; it += RF_T) {
+        const int tr = it / per, j = it - tr * per;
+        const int ia = 2 * (p0 + tr), ib = ia + 1;
+        const C v = z[tr * L + j];
+        const bool zero = ydst && (j == 0 || j == M);
+        const TG xa = zero ? TG(0) : (ydst ? -v.y : v.x);
+        const TG xb = zero ? TG(0) : (ydst ? v.x : v.y);
+        emit_output(out, f, (size_t)ia * N + j, xa, write_full);
+        if (ib < N) emit_output(out, f, (size_t)ib * N + j, xb, write_full);
+    }
+    if (f == F_BZ) {
+        // Row sums of the raw Bz in a fixed order: one wave per row, lanes
+        // over j, a butterfly reduction (z is not written after the FFT).
+        const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
+        for (int r = wave; r < 2 * ntr; r += RF_T / 64) {
+            const int tr = r >> 1, part = r & 1, row = 2 * (p0 + tr) + part;
+            double s = 0;
+            for (int j = lane; j <= M; j += 64) {
+                const C v = z[tr * L + j];
+                const bool zero = ydst && (j == 0 || j == M);
+                const TG x = zero ? TG(0) : (part ? (ydst ? v.x : v.y) : (ydst ? -v.y : v.x));
+                s += (double)x;
+            }
+#pragma unroll
+            for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
+            if (lane == 0 && row < N) g.rowsum[row] = s;
+        }
+    }
+}
+
+template <typename TG> struct RfCaps;
+// 16, 32 and 64 KB of LDS per block.
+template <> struct RfCaps<float> { static constexpr int c0 = 2048, c1 = 4096, c2 = 8192; };
+template <> struct RfCaps<double> { static constexpr int c0 = 1024, c1 = 2048, c2 = 4096; };
+
+bool rfft_cap_supported(bool fp32, int cap)
+{
+    return fp32 ? (cap == RfCaps<float>::c0 || cap == RfCaps<float>::c1 || cap == RfCaps<float>::c2)
+                : (cap == RfCaps<double>::c0 || cap == RfCaps<double>::c1 ||
+                   cap == RfCaps<double>::c2);
+}
+
+template <typename TG, typename TPRED, typename TFULL, typename THALF, int CAP>
+void rf_launch(const RfftGeom &g, const RhsEval<TG, TPRED> &rhs, const SolveOut<TFULL, THALF> &out,
+               bool write_full, hipStream_t s)
+{
+    const dim3 grid(((g.N + 1) / 2 + g.tpb - 1) / g.tpb, 7);
+    k_rf_rows_fwd<TG, TPRED, CAP><<<grid, RF_T, 0, s>>>(g, rhs);
+    k_rf_cols<TG, CAP><<<grid, RF_T, 0, s>>>(g);
+    k_rf_rows_inv<TG, TFULL, THALF, CAP><<<grid, RF_T, 0, s>>>(g, out, write_full);
+}
+
+template <typename TG, typename TPRED, typename TFULL, typename THALF>
+hipError_t launch_compute_fields_rfft(const RfftGeom &g, const SolveIn<TG, TPRED> &in,
+                                      const SolveOut<TFULL, THALF> &out, bool write_full,
+                                      hipStream_t s, KernelTimer *timer)
+{
+    const int N = g.N, L = 2 * (N - 1);
+    if (N < 9 || g.tpb < 1 || 2 * (int64_t)g.tpb * L > g.cap ||
+        !rfft_cap_supported(sizeof(TG) == 4, g.cap))
+        return hipErrorInvalidValue;
+    const RhsEval<TG, TPRED> rhs{in, N, TG(1.0 / (2.0 * g.h)), TG(1.0 / g.xi_step), TG(g.c)};
+    hipError_t e;
+    if (timer && (e = timer->begin(KC_SOLVE_GEMM, 3, s)) != hipSuccess) return e;
+    if (g.cap == RfCaps<TG>::c0)
+        rf_launch<TG, TPRED, TFULL, THALF, RfCaps<TG>::c0>(g, rhs, out, write_full, s);
+    else if (g.cap == RfCaps<TG>::c1)
+        rf_launch<TG, TPRED, TFULL, THALF, RfCaps<TG>::c1>(g, rhs, out, write_full, s);
+    else
+        rf_launch<TG, TPRED, TFULL, THALF, RfCaps<TG>::c2>(g, rhs, out, write_full, s);
+    if (timer && (e = timer->end(s)) != hipSuccess) return e;
+    const int fb = (int)std::min<int64_t>(1024, ((int64_t)N * N + 255) / 256);
+    k_bz_finalize<TFULL, THALF><<<fb, 256, 0, s>>>(N, g.rowsum, out.prev[F_BZ], out.full[F_BZ],
+                                                   out.half[F_BZ]);
+    return hipGetLastError();
+}
+
+template hipError_t launch_compute_fields_rfft<double, double, double, double>(
+    const RfftGeom &, const SolveIn<double, double> &, const SolveOut<double, double> &, bool,
+    hipStream_t, KernelTimer *);
+template hipError_t launch_compute_fields_rfft<float, double, double, float>(
+    const RfftGeom &, const SolveIn<float, double> &, const SolveOut<double, float> &, bool,
+    hipStream_t, KernelTimer *);
+template hipError_t launch_compute_fields_rfft<float, float, double, float>(
+    const RfftGeom &, const SolveIn<float, float> &, const SolveOut<double, float> &, bool,
+    hipStream_t, KernelTimer *);
+
+}  // namespace lc3d

@@ -22,6 +22,25 @@ __device__ __forceinline__ bool in_rhs_domain(int f, int i, int j, int N) {
     return (!dst_along_x(f) || (i > 0 && i < N - 1)) && (!dst_along_y(f) || (j > 0 && j < N - 1));
 }
 
+struct ScaleGeom {
+    const double *lam;
+    double c;      // subtraction coefficient of the mixed fields
+    double d2;     // (2 (N - 1))^2
+};
+
+// The solver-matrix factor of field f at spectral indices (kx, ky) = (gr, gc)
+// (initialization.py:15-67): zero where a DST index is an edge and at [0, 0]
+// for the Neumann field Bz.
+__device__ __forceinline__ double solver_factor(int f, int gr, int gc, int N, const ScaleGeom &sg)
+{
+    const int M = N - 1;
+    const bool dx = dst_along_x(f), dy = dst_along_y(f);
+    const double cc = (f == F_EZ || f == F_BZ || f == F_PHI) ? 0.0 : sg.c;
+    const bool ok = (!dx || (gr > 0 && gr < M)) && (!dy || (gc > 0 && gc < M)) &&
+                    !(f == F_BZ && gr == 0 && gc == 0);
+    return ok ? (1.0 / (sg.lam[gr] + sg.lam[gc] + cc)) / sg.d2 : 0.0;
+}
+
 // Right-hand side of field f at (i, j) from the currents of layers k (P) and
 // k+1 (F), the beam densities and the predictor fields (fields.py:112-266).
 // Valid on in_rhs_domain(); the x/y stencils stay inside the grid there.

@@ -34,11 +34,6 @@ template <typename T> struct GemmBatch {
     T *C[7];
 };
 
-struct ScaleGeom {
-    const double *lam;
-    double c;      // subtraction coefficient of the mixed fields
-    double d2;     // (2 (N - 1))^2
-};
 
 // One thread per grid point evaluates the seven fields' right-hand sides
 // there (the field loop unrolled, so the currents they share are loaded
@@ -53,19 +48,6 @@ __global__ __launch_bounds__(256) void k_rhs(RhsEval<TG, TPRED> rhs, TG *__restr
     const int i = (int)(o / N), j = (int)(o % N);
 #pragma unroll
     for (int f = 0; f < 7; ++f) work[f * nn + o] = in_rhs_domain(f, i, j, N) ? rhs(f, i, j) : TG(0);
-}
-
-// The solver-matrix factor of field f at spectral indices (kx, ky) = (gr, gc)
-// (initialization.py:15-67): zero where a DST index is an edge and at [0, 0]
-// for the Neumann field Bz.
-__device__ __forceinline__ double solver_factor(int f, int gr, int gc, int N, const ScaleGeom &sg)
-{
-    const int M = N - 1;
-    const bool dx = dst_along_x(f), dy = dst_along_y(f);
-    const double cc = (f == F_EZ || f == F_BZ || f == F_PHI) ? 0.0 : sg.c;
-    const bool ok = (!dx || (gr > 0 && gr < M)) && (!dy || (gc > 0 && gc < M)) &&
-                    !(f == F_BZ && gr == 0 && gc == 0);
-    return ok ? (1.0 / (sg.lam[gr] + sg.lam[gc] + cc)) / sg.d2 : 0.0;
 }
 
 constexpr int GBM = 64, GBN = 64;   // output tile

@@ -81,13 +81,15 @@ def context(config: Config, max_particles: int | None = None) -> _native.Context
     radix = tuple(int(r) for r in radix) if radix else None
     dev = device().index
     solver = config.field_solver()
-    key = (dev, prec, radix, solver) + tuple(sorted(g.items()))
+    shape = (int(config.get('rfft-cap', 0) or 0), int(config.get('rfft-tpb', 0) or 0))
+    key = (dev, prec, radix, solver, shape) + tuple(sorted(g.items()))
     with _LOCK:
         ctx = _CACHE.get(key)
         if ctx is None or ctx.max_particles < cap:
             with torch.cuda.device(dev):
                 ctx = _native.Context(eigenvalues=eigenvalues(g['N'], g['h']), precision=prec,
                                       max_particles=cap, fft_radix=radix,
-                                      field_solver=solver, **g)
+                                      field_solver=solver, rfft_cap=shape[0],
+                                      rfft_tpb=shape[1], **g)
             _CACHE[key] = ctx
     return ctx

@@ -1,8 +1,8 @@
 // Drives every entry point of the C ABI (include/lcodepy_amd.h) on the CPU
-// emulation, in both precision forms and with both field solvers, with
+// emulation, in both precision forms and with every field solver, with
 // exactly sized host buffers: field solve, deposit, push, fused plasma step
 // (1 and 2 sorts) and the beam kernels (including off-grid and non-finite
-// beam particles); then checks that the matrix and FFT solves agree.
+// beam particles); then checks that the FFT solves agree with the matrix one.
 #include "hip/hip_runtime.h"
 #include "../../include/lcodepy_amd.h"
 #include <random>
@@ -95,7 +95,7 @@ template <typename TP, typename TG, typename TH> void run_all(int N, int prec, i
     double sec[3]; int64_t nl[3];
     CHECK(lc3d_kernel_times(r.ctx, sec, nl));
     CHECK(lc3d_kernel_timing(r.ctx, 0));
-    const int64_t want_gemm = solver == LC3D_SOLVER_MATRIX ? 16 : 0;
+    const int64_t want_gemm = solver == LC3D_SOLVER_MATRIX ? 16 : solver == LC3D_SOLVER_RFFT ? 12 : 0;
     if (nl[0] != 6 || nl[1] != 6 || nl[2] != want_gemm) { printf("FAIL kernel launch counts %lld %lld %lld\n", (long long)nl[0], (long long)nl[1], (long long)nl[2]); exit(1); }
     printf("plasma_step 1 sort ok (timed launches %lld %lld %lld)\n", (long long)nl[0], (long long)nl[1], (long long)nl[2]); fflush(stdout);
     Sort<TP> ions(r, 0.3, 0.01); ions.s.column = 1;
@@ -120,9 +120,9 @@ template <typename TP, typename TG, typename TH> void run_all(int N, int prec, i
 
 // The same solve through both implementations; returns max |a - b| / max |b|
 // over the seven full and half fields.
-template <typename TG, typename TH> double cross_check(int N, int prec) {
+template <typename TG, typename TH> double cross_check(int N, int prec, int solver) {
     size_t nn = (size_t)N * N;
-    Run a(N, prec, LC3D_SOLVER_MATRIX), b(N, prec, LC3D_SOLVER_FFT);
+    Run a(N, prec, LC3D_SOLVER_MATRIX), b(N, prec, solver);
     lc3d_fields pred = fields<double>(nn, 0.01), prev = fields<double>(nn, 0.01);
     lc3d_currents cp = currents<TG>(N, 0.1), cf = currents<TG>(N, 0.1);
     double *rbf = arr<double>(nn, 0.05), *rbp = arr<double>(nn, 0.05);
@@ -149,14 +149,17 @@ template <typename TG, typename TH> double cross_check(int N, int prec) {
 int main(int argc, char **argv) {
     for (int i = 1; i < argc; ++i) {
         int N = atoi(argv[i]);
-        for (int solver : {LC3D_SOLVER_MATRIX, LC3D_SOLVER_FFT}) {
+        for (int solver : {LC3D_SOLVER_MATRIX, LC3D_SOLVER_FFT, LC3D_SOLVER_RFFT}) {
             run_all<double, double, double>(N, LC3D_FP64, solver);
             run_all<float, float, float>(N, LC3D_FP32_BULK, solver);
         }
-        const double e64 = cross_check<double, double>(N, LC3D_FP64);
-        const double e32 = cross_check<float, float>(N, LC3D_FP32_BULK);
-        printf("matrix vs FFT solve N=%d: fp64 %.3e, fp32-bulk %.3e\n", N, e64, e32); fflush(stdout);
-        if (!(e64 < 1e-12) || !(e32 < 1e-4)) { printf("FAIL solver mismatch\n"); return 1; }
+        for (int solver : {LC3D_SOLVER_FFT, LC3D_SOLVER_RFFT}) {
+            const double e64 = cross_check<double, double>(N, LC3D_FP64, solver);
+            const double e32 = cross_check<float, float>(N, LC3D_FP32_BULK, solver);
+            printf("matrix vs solver %d N=%d: fp64 %.3e, fp32-bulk %.3e\n", solver, N, e64, e32);
+            fflush(stdout);
+            if (!(e64 < 1e-12) || !(e32 < 1e-4)) { printf("FAIL solver mismatch\n"); return 1; }
+        }
     }
     printf("ALL OK\n");
 }

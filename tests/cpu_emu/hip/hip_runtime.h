@@ -28,6 +28,15 @@ struct alignas(16) int4 { int x, y, z, w; };
 typedef int hipError_t;
 typedef void *hipStream_t;
 #define hipSuccess 0
+#define hipErrorInvalidValue 1
+// Index checks the kernels compile in for this build only.
+#define RF_CHECK(cond)                                                                  \
+    do {                                                                                \
+        if (!(cond)) {                                                                  \
+            std::fprintf(stderr, "RF_CHECK failed: %s (%s:%d)\n", #cond, __FILE__, __LINE__); \
+            std::abort();                                                               \
+        }                                                                               \
+    } while (0)
 inline hipError_t hipGetLastError() { return 0; }
 inline const char *hipGetErrorString(hipError_t) { return "emu"; }
 

@@ -36,10 +36,12 @@ def _env():
 
 @pytest.mark.parametrize('N', [33, 65, 129])
 def test_abi_entry_points_in_bounds(emu_build, N):
-    """Every entry point in both precision forms and both field solvers
-    (matrix and LDS FFT), one and two sorts, at N = 33 and at the two sizes
-    whose LDS-FFT solve faulted on the GPU (65, 129).  N = 513 is beyond
-    what the fiber emulation runs in minutes."""
+    """Every entry point in both precision forms and every field solver
+    (matrix, the switched-off length-(N-1) FFT, and the paired LDS FFT with
+    its index checks), one and two sorts, at N = 33 and at the two sizes
+    whose length-(N-1) FFT solve faulted on the GPU (65, 129).  N = 513 is
+    beyond what the fiber emulation runs in minutes here (the paired FFT's
+    solve alone is checked there through run_parity.py)."""
     r = subprocess.run([os.path.join(emu_build, 'abi_check'), str(N)], capture_output=True,
                        text=True, env=_env(), timeout=600)
     assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-4000:]
@@ -48,9 +50,15 @@ def test_abi_entry_points_in_bounds(emu_build, N):
 
 
 def test_gpu_parity_suite_on_cpu_build(emu_build):
-    ids = ['tests/test_gpu_kernels.py::test_compute_fields_matches_oracle[33-fp64]',
-           'tests/test_gpu_kernels.py::test_compute_fields_matches_oracle[97-fp64]',
-           'tests/test_gpu_kernels.py::test_compute_fields_matches_oracle[65-fp32-bulk]',
+    ids = ['tests/test_gpu_kernels.py::test_compute_fields_matches_oracle[33-fp64-matrix]',
+           'tests/test_gpu_kernels.py::test_compute_fields_matches_oracle[97-fp64-matrix]',
+           'tests/test_gpu_kernels.py::test_compute_fields_matches_oracle[65-fp32-bulk-matrix]',
+           'tests/test_gpu_kernels.py::test_compute_fields_matches_oracle[97-fp64-rfft]',
+           'tests/test_gpu_kernels.py::test_compute_fields_matches_oracle[113-fp64-rfft]',
+           'tests/test_gpu_kernels.py::test_compute_fields_matches_oracle[65-fp32-bulk-rfft]',
+           'tests/test_gpu_kernels.py::test_rfft_plans_and_launch_shapes[65-fp64-plan4-1024-3]',
+           'tests/test_gpu_kernels.py::test_rfft_plans_and_launch_shapes[65-fp32-bulk-plan5-2048-7]',
+           'tests/test_gpu_kernels.py::test_rfft_plans_and_launch_shapes[97-fp64-plan6-4096-5]',
            'tests/test_gpu_kernels.py::test_deposit_matches_oracle_and_is_deterministic[65-fp64]',
            'tests/test_gpu_kernels.py::test_deposit_matches_oracle_and_is_deterministic[65-fp32-bulk]',
            'tests/test_gpu_kernels.py::test_push_matches_oracle[65-fp64]',

@@ -4,7 +4,7 @@ reference's host-side strategy would read from the same states
 points, every xi step, from a run with mobile ions and an evolving beam.
 Grid values and maxima are copies, so they must agree exactly; the energy
 flux Sf is checked against the oracle's restatement of
-calculate_energy_fluxes (1e-12)."""
+calculate_energy_fluxes (1e-12 of the largest flux of the run)."""
 import numpy as np
 import pytest
 
@@ -76,7 +76,10 @@ def test_fxi_probes_match_host_reads(precision, tmp_path, monkeypatch):
         got = out[k]
         assert got.shape == want.shape, k
         if k == 'Sf':
-            assert np.all(np.abs(got - want) <= 1e-12 * np.abs(want)), k
+            # relative to the largest flux of the run: the first steps' flux
+            # (1e-8) is a sum of kinetic terms sqrt(m^2 + p^2) - m whose
+            # rounding (a few 1e-16 here) does not scale with it
+            assert np.max(np.abs(got - want)) <= 1e-12 * np.max(np.abs(want)), k
         else:
             assert np.array_equal(got, want), k
     assert np.max(np.abs(out['Ez'])) > 0 and np.max(np.abs(out['rho_beam'])) > 0

@@ -1,11 +1,13 @@
 """End-to-end runs of the HIP path against the reference's golden fixtures
 and against the oracle step by step.
 
-Tolerance (float64 form): the reference compares its own CPU runs at
-rtol 5e-8 (tests/integration/test_3D_part.py:7); the HIP path sums the
-deposits cell by cell and fuses multiply-adds, so after 201-505 xi steps it
-is compared as max |gpu - ref| / max |ref| <= 1e-6 per quantity (the
-on-axis Ez bound the float64 GPU form is held to)."""
+Tolerance (float64 form): the reference compares its own CPU runs element
+by element at rtol 5e-8, atol 1e-125 (tests/integration/test_3D_part.py:7,
+50-58); the fixtures are held to that same element-wise check and, as a
+norm, to max |gpu - ref| / max |ref| <= 1e-11 per quantity (measured at most
+7.3e-13, the Bz of 3D_test1).  The HIP path sums the deposits cell by cell
+and fuses multiply-adds, which is what separates it from the reference's
+serial float64 loops at the 1e-13 level."""
 import os
 
 import numpy as np
@@ -33,21 +35,27 @@ def test_simulation_matches_reference_fixture(which, golden_dir):
     particles, fields, currents = sim.push_solver._plasmastate
     el = particles['electrons']
     golden = np.load(os.path.join(golden_dir, f'3D_{which}.npz'))
-    worst = {}
+    got = {}
     for k in ('x_offt', 'y_offt', 'px', 'py', 'pz'):
-        worst[k] = relerr(getattr(el, k).cpu().numpy()[::10, ::10], golden[k])
+        got[k] = getattr(el, k).cpu().numpy()[::10, ::10]
     for k in ('Ex', 'Ey', 'Ez', 'Bx', 'By', 'Bz'):
-        worst[k] = relerr(getattr(fields, k).cpu().numpy()[::10, ::10], golden[k])
+        got[k] = getattr(fields, k).cpu().numpy()[::10, ::10]
     for k in ('ro', 'jx', 'jy', 'jz'):
-        worst[k] = relerr(getattr(currents, k).cpu().numpy()[:, ::10, ::10].sum(axis=0), golden[k])
+        got[k] = getattr(currents, k).cpu().numpy()[:, ::10, ::10].sum(axis=0)
+    worst = {k: relerr(v, golden[k]) for k, v in got.items()}
+    # the reference's own element-wise check of its fixtures
+    close = {k: bool(np.allclose(v, golden[k], rtol=5e-8, atol=1e-125)) for k, v in got.items()}
     print(which, {k: f'{v:.1e}' for k, v in worst.items()})
+    print(which, 'np.allclose(rtol=5e-8, atol=1e-125):', close)
     for k, v in worst.items():
-        assert v < 1e-6, (k, v)
+        assert v < 1e-11, (k, v)
+    assert all(close.values()), close
 
 
 def test_on_axis_ez_matches_oracle_every_step():
     """129 x 129 window, 4 ppc, a generated beam, 101 xi steps: the on-axis
-    Ez of every step within 1e-6 of the oracle's (normalised by max |Ez|)."""
+    Ez of every step within 1e-12 of the oracle's (normalised by max |Ez|;
+    measured 1.1e-15)."""
     from lcodepy_amd.beam_generator import generate_beam
     from lcodepy_amd.simulation import Simulation
     cfg = fixture_config('beam_evol')
@@ -64,7 +72,7 @@ def test_on_axis_ez_matches_oracle_every_step():
     assert ez_gpu.shape == ez_oracle.shape
     err = np.max(np.abs(ez_gpu - ez_oracle)) / np.max(np.abs(ez_oracle))
     print(f'N={p.N}: on-axis Ez max error {err:.2e} over {len(ez_gpu)} xi steps')
-    assert err < 1e-6
+    assert err < 1e-12
 
 
 def test_fp32_bulk_form_tracks_oracle():
@@ -130,9 +138,13 @@ def test_beam_evolution_matches_oracle():
     order_o = np.argsort(beam_o['id'])
     print(f'N={p.N}: {len(ids)} drained particles (oracle {len(beam_o["id"])})')
     assert np.array_equal(ids[order_g], beam_o['id'][order_o])
+    errs = {}
     for k in ('x', 'y', 'xi', 'px', 'py', 'pz'):
         g = getattr(drained, k).cpu().numpy()[order_g]
         o = beam_o[k][order_o]
-        assert relerr(g, o) < 1e-9, k
+        errs[k] = relerr(g, o)
     ez_g = np.array([float(v) for v in sim.push_solver.ez_axis])
-    assert relerr(ez_g, ez_o) < 1e-6
+    errs['ez_axis'] = relerr(ez_g, ez_o)
+    print('drained beam and last on-axis Ez vs oracle:', {k: f'{v:.1e}' for k, v in errs.items()})
+    for k, v in errs.items():
+        assert v < 1e-9, (k, v)
