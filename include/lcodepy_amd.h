@@ -220,6 +220,23 @@ int lc3d_energy_flux(lc3d_ctx *ctx, const lc3d_fields6 *fields, int64_t n, const
                      const void *px, const void *py, const double *pz, double *psi,
                      void *stream);
 
+/* Allocation-free form of lc3d_energy_flux.  It has no counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, serving loops).  It needs no context: N, h and
+ * the precision of m, px, py (LC3D_FP64 or LC3D_FP32_BULK) are arguments,
+ * and the partial sums live in a static device array of the library
+ * (2 x LC3D_NOALLOC_FLUX_BLOCKS float64 = 16 KiB per device, kept until the
+ * process ends), so the call allocates nothing.  Limits: N <= 1025 and
+ * n <= 2^31 - 1; beyond them it returns LC3D_EINVAL (message from
+ * lc3d_last_error(NULL)).  The result equals lc3d_energy_flux's bit for bit.
+ * Every call on a device shares that device's array: calls that overlap on
+ * one device (other streams or threads) race, so order them on one stream. */
+#define LC3D_NOALLOC_FLUX_BLOCKS 1024
+#define LC3D_NOALLOC_FLUX_MAX_N 1025
+#define LC3D_NOALLOC_FLUX_MAX_PARTICLES 2147483647LL
+int lc3d_energy_flux_noalloc(int32_t N, double h, int32_t precision, const lc3d_fields6 *fields,
+                             int64_t n, const void *m, const void *px, const void *py,
+                             const double *pz, double *psi, void *stream);
 /* ---- f(xi) probes ------------------------------------------------------------------ */
 #define LC3D_PROBE_MAX_SOURCES 16
 typedef struct lc3d_probe_sources {

@@ -85,7 +85,7 @@ SYMBOLS = ('lc3d_create', 'lc3d_destroy', 'lc3d_last_error', 'lc3d_precision',
            'lc3d_error_flags', 'lc3d_kernel_timing', 'lc3d_kernel_times', 'lc3d_move_wo_fields', 'lc3d_move_smart', 'lc3d_deposit',
            'lc3d_compute_fields', 'lc3d_plasma_step', 'lc3d_deposit_beam',
            'lc3d_beam_substep', 'lc3d_move_beam', 'lc3d_energy_flux', 'lc3d_probe_sample',
-           'lc3d_abs_max')
+           'lc3d_abs_max', 'lc3d_energy_flux_noalloc')
 
 _LIB = None
 _LOCK = threading.Lock()
@@ -119,6 +119,8 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
     L.lc3d_deposit_beam.argtypes = [_vp, i64, i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp]
     L.lc3d_beam_substep.argtypes = [_vp, i64, _vp, _vp, dbl, dbl, _vp, _vp, _vp]
     L.lc3d_energy_flux.argtypes = [_vp, P(Fields6), i64, _vp, _vp, _vp, _vp, _vp, _vp]
+    L.lc3d_energy_flux_noalloc.argtypes = [i32, dbl, i32, P(Fields6), i64, _vp, _vp, _vp, _vp,
+                                           _vp, _vp]
     L.lc3d_probe_sample.argtypes = [_vp, P(ProbeSources), i32, _vp, _vp, _vp, _vp]
     L.lc3d_abs_max.argtypes = [_vp, _vp, i64, i64, i64, i64, i64, i64, _vp, _vp]
     L.lc3d_move_beam.argtypes = [_vp, i64, i64, _vp, _vp, P(Fields6), P(Fields6), _vp, _vp,

@@ -15,6 +15,8 @@ using namespace lc3d;
 
 // Blocks of the energy-flux reduction (each leaves two float64 partials).
 constexpr int DIAG_BLOCKS = 1024;
+static_assert(DIAG_BLOCKS == LC3D_NOALLOC_FLUX_BLOCKS,
+              "the allocation-free energy flux sums over the same blocks");
 
 struct lc3d_ctx {
     lc3d_params prm{};
@@ -564,7 +566,10 @@ void lc3d_destroy(lc3d_ctx *c)
     delete c;
 }
 
-const char *lc3d_last_error(const lc3d_ctx *c) { return c ? c->last_error.c_str() : "null context"; }
+const char *lc3d_last_error(const lc3d_ctx *c)
+{
+    return c ? c->last_error.c_str() : noalloc_last_error();
+}
 
 int lc3d_precision(const lc3d_ctx *c) { return c ? c->prm.precision : -1; }
 

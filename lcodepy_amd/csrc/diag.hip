@@ -8,12 +8,23 @@
 // and particles into two float64 partials (a fixed-order tree in LDS), and
 // one block adds the partials in a fixed order, so the result is
 // reproducible bit for bit.
+#include <mutex>
+#include <string>
+
+#include "../../include/lcodepy_amd.h"
 #include "common.h"
 #include "kernels.h"
 
 namespace lc3d {
 
 constexpr int DIAG_THREADS = 256;
+
+// Partial sums of the allocation-free energy flux (lc3d_energy_flux_noalloc):
+// two float64 per block of the first launch, at most NOALLOC_BLOCKS blocks
+// (16 KiB per device, held until the process ends).  Host code reaches it
+// only through the address hipGetSymbolAddress returns on each device.
+constexpr int NOALLOC_BLOCKS = LC3D_NOALLOC_FLUX_BLOCKS;
+static __device__ __align__(256) double scratch[2 * NOALLOC_BLOCKS];
 
 __device__ __forceinline__ double block_sum(double v, double *red)
 {
@@ -94,3 +105,77 @@ template hipError_t launch_energy_flux<float>(int, double, const Fields6<double>
                                               const double *, double *, int, double *, hipStream_t);
 
 }  // namespace lc3d
+
+namespace {
+
+constexpr int MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+double *g_scratch[MAX_DEVICES] = {};
+thread_local std::string g_noalloc_error = "null context";
+
+int noalloc_fail(int code, const std::string &msg)
+{
+    g_noalloc_error = msg;
+    return code;
+}
+
+// The device address of `scratch` on the current device, looked up on the
+// first call there.
+hipError_t scratch_on_device(double **out)
+{
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    if (dev < 0 || dev >= MAX_DEVICES) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch[dev]) {
+        void *p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(lc3d::scratch));
+        if (e != hipSuccess) return e;
+        g_scratch[dev] = static_cast<double *>(p);
+    }
+    *out = g_scratch[dev];
+    return hipSuccess;
+}
+
+}  // namespace
+
+const char *lc3d::noalloc_last_error() { return g_noalloc_error.c_str(); }
+
+extern "C" int lc3d_energy_flux_noalloc(int32_t N, double h, int32_t precision,
+                                        const lc3d_fields6 *f, int64_t n, const void *m,
+                                        const void *px, const void *py, const double *pz,
+                                        double *psi, void *stream)
+{
+    using namespace lc3d;
+    if (N < 1 || N > LC3D_NOALLOC_FLUX_MAX_N || n < 0 || n > LC3D_NOALLOC_FLUX_MAX_PARTICLES)
+        return noalloc_fail(LC3D_EINVAL, "energy_flux_noalloc: N must be in [1, " +
+                                             std::to_string(LC3D_NOALLOC_FLUX_MAX_N) +
+                                             "] and n in [0, " +
+                                             std::to_string((long long)LC3D_NOALLOC_FLUX_MAX_PARTICLES) +
+                                             "]");
+    if ((precision != LC3D_FP64 && precision != LC3D_FP32_BULK) || !f || !f->Ex || !f->Ey ||
+        !f->Ez || !f->Bx || !f->By || !f->Bz || !psi || (n > 0 && (!m || !px || !py || !pz)))
+        return noalloc_fail(LC3D_EINVAL, "energy_flux_noalloc: bad arguments");
+    double *part = nullptr;
+    hipError_t e = scratch_on_device(&part);
+    if (e != hipSuccess)
+        return noalloc_fail(LC3D_EHIP, std::string("energy_flux_noalloc: scratch address: ") +
+                                           hipGetErrorString(e));
+    const Fields6<double> fl{static_cast<const double *>(f->Ex), static_cast<const double *>(f->Ey),
+                             static_cast<const double *>(f->Ez), static_cast<const double *>(f->Bx),
+                             static_cast<const double *>(f->By), static_cast<const double *>(f->Bz)};
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    if (precision == LC3D_FP32_BULK)
+        e = launch_energy_flux<float>(N, h, fl, n, static_cast<const float *>(m),
+                                      static_cast<const float *>(px), static_cast<const float *>(py),
+                                      pz, part, NOALLOC_BLOCKS, psi, st);
+    else
+        e = launch_energy_flux<double>(N, h, fl, n, static_cast<const double *>(m),
+                                       static_cast<const double *>(px),
+                                       static_cast<const double *>(py), pz, part, NOALLOC_BLOCKS,
+                                       psi, st);
+    if (e != hipSuccess)
+        return noalloc_fail(LC3D_EHIP, std::string("energy_flux_noalloc: ") + hipGetErrorString(e));
+    return LC3D_OK;
+}

@@ -241,6 +241,10 @@ struct ProbeSources {
     const void *src[PROBE_MAX_SOURCES];
     int is_f32[PROBE_MAX_SOURCES];
 };
+// The message of the last failed call that has no context (the
+// allocation-free entry points), for lc3d_last_error(NULL); per host thread.
+const char *noalloc_last_error();
+
 hipError_t launch_probe_sample(const ProbeSources &ps, int N, int n_points, const int *ix,
                                const int *iy, double *out, hipStream_t s);
 

@@ -115,6 +115,15 @@ template <typename TP, typename TG, typename TH> void run_all(int N, int prec, i
     CHECK(lc3d_move_beam(r.ctx, nb, 3, bqm, bdt, &fk, &fk, bst, bid, bx, by, bxi, bpx, bpy, bpz, l, mv, fl, nullptr));
     uint32_t flags = 0; CHECK(lc3d_error_flags(r.ctx, &flags, nullptr));
     printf("beam ok, error flags %u\n", flags); fflush(stdout);
+    // energy flux: the context entry and the allocation-free one (static scratch)
+    double *psi = arr<double>(2);
+    CHECK(lc3d_energy_flux(r.ctx, &fk, r.n, s.m, s.px, s.py, s.pz, psi, nullptr));
+    CHECK(lc3d_energy_flux_noalloc(N, r.h, prec, &fk, r.n, s.m, s.px, s.py, s.pz, psi + 1, nullptr));
+    CHECK(lc3d_energy_flux_noalloc(N, r.h, prec, &fk, r.n / 3, s.m, s.px, s.py, s.pz, psi + 1, nullptr));
+    CHECK(lc3d_energy_flux(r.ctx, &fk, r.n / 3, s.m, s.px, s.py, s.pz, psi, nullptr));
+    if (!(psi[0] == psi[1]) || !(psi[0] > 0)) { printf("FAIL energy flux noalloc %.17g vs %.17g\n", psi[1], psi[0]); exit(1); }
+    if (lc3d_energy_flux_noalloc(1026, r.h, prec, &fk, 0, nullptr, nullptr, nullptr, nullptr, psi, nullptr) != LC3D_EINVAL) { printf("FAIL noalloc size check\n"); exit(1); }
+    printf("energy flux ok (allocation-free form equal)\n"); fflush(stdout);
     lc3d_destroy(r.ctx);
 }
 
