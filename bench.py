@@ -38,10 +38,18 @@ HBM_PEAK = 8.0e12
 MFMA_PEAK = {'fp32-bulk': 157.3e12, 'fp64': 78.6e12}   # dense matrix-core FLOP/s
 WINDOW_STEPS = 1001                                     # xi steps of one time step
 TRAFFIC_FILE = os.path.join(ROOT, 'profiles', 'xi_step_513_4ppc_hbm_traffic.md')
-LAUNCHES_PER_STEP = {'push': 3, 'deposit_gather': 3, 'solve_gemm': 8}
+# Launches of each timed class per xi step (one plasma sort): 3 smart
+# pushes, 3 deposits, and per solve 4 transform GEMMs (matrix) or 3 FFT
+# kernels (rfft), 2 solves.
+LAUNCHES_PER_STEP = {'push': 3, 'deposit': 3, 'solve': {'matrix': 8, 'rfft': 6}}
+# Algorithmic words per particle per xi step (46: 9 + 2 x 11 reads, 3 x 5
+# writes) and per grid point (166: 3 pushes x 10 N^2, 2 solves x 68 N^2).
+WORDS_PER_PARTICLE = 46
+WORDS_PER_GRID_POINT = 166
+SOLVE_WORDS_PER_GRID_POINT = 68
 
 
-def workload_config(precision: str, field_solver: str = 'matrix') -> dict:
+def workload_config(precision: str, field_solver: str = 'auto') -> dict:
     return {'geometry': '3d', 'processing-unit-type': 'mi355x', 'precision': precision,
             'window-width': 12.8, 'transverse-step': 0.05,
             'window-length': 20.0, 'xi-step': 0.02, 'time-limit': 1.0, 'time-step': 1.0,
@@ -97,7 +105,7 @@ def thermalize(particles: dict, m: float, seed: int = 2, sigma: float = 1e-3) ->
 class Run:
     """One precision form of the workload, stepped one xi step at a time."""
 
-    def __init__(self, precision: str, n_beam: int, field_solver: str = 'matrix'):
+    def __init__(self, precision: str, n_beam: int, field_solver: str = 'auto'):
         from lcodepy_amd.beam3d.beam_io import BeamDrain, BeamSource
         from lcodepy_amd.config import Config
         from lcodepy_amd.plasma3d.initialization import init_plasma
@@ -145,6 +153,22 @@ def live_kernel_times(run: Run, steps: int) -> dict:
     return times
 
 
+def step_bytes(n: int, N: int, precision: str) -> int:
+    """Algorithmic HBM bytes of one xi step, B_step = Np b_p + 166 N^2 b_g:
+    46 words per particle, 368 B in float64 or 272 B in FP32-bulk (x_offt,
+    y_offt, x_init, y_init and pz float64, the rest float32), and 166 words
+    per grid point of 8 or 4 B."""
+    if precision == 'fp64':
+        return n * WORDS_PER_PARTICLE * 8 + WORDS_PER_GRID_POINT * N * N * 8
+    return n * 272 + WORDS_PER_GRID_POINT * N * N * 4
+
+
+def solve_bytes(N: int, precision: str) -> int:
+    """Algorithmic bytes of one field solve: 33 N^2 words of inputs and
+    outputs and 7 fields x 5 N^2 of transform passes (68 N^2 words)."""
+    return SOLVE_WORDS_PER_GRID_POINT * N * N * (8 if precision == 'fp64' else 4)
+
+
 def gemm_flops(N: int) -> float:
     """Flops of one solve-GEMM launch: the seven N x N products (one per
     field) of a transform pass, 2 N^3 each."""
@@ -166,9 +190,10 @@ def hbm_traffic(precision: str, kernel_class: str):
 
 
 def deposit_bytes(n: int, N: int, bulk: int) -> int:
-    """Algorithmic bytes of one deposit (weights.py:35-67): the nine particle
-    arrays read (x_init y_init x_offt y_offt pz float64, px py q m of the
-    bulk type) and the four current grids written."""
+    """Algorithmic bytes of one deposit not fused with its push
+    (weights.py:35-67): the nine particle arrays read (x_init y_init x_offt
+    y_offt pz float64, px py q m of the bulk type) and the four current grids
+    written.  The timed deposit is the whole cell sort and gather."""
     return n * (5 * 8 + 4 * bulk) + 4 * N * N * bulk
 
 
@@ -185,22 +210,18 @@ def push_bytes(n: int, N: int, bulk: int) -> float:
     return (first + 2 * later) / 3
 
 
-def cpu_baseline(sample_steps: int) -> dict:
-    """The oracle (float64 numpy + C, this repository's CPU restatement) on
-    the same workload: pushes on all host cores (OpenMP, as the reference's
-    Numba prange), deposit serial (as weights.py:35), scipy.fftpack DST/DCT."""
-    from lcodepy_amd.config import Config
+def _oracle_sample(p, cfg, steps: int, start_step: int) -> float:
+    """Seconds of `steps` xi steps of the oracle's xi loop (step_dt) from
+    xi step start_step + 1 on, with the driver's layers deposited and moved:
+    the plasma state is the initial one with thermal momenta, the beam the
+    part of the driver behind the start."""
     from oracle import lcode_oracle as O
-    cfg = Config(workload_config('fp64'))
-    cfg._adjust_config_values_3d()
-    cores = os.cpu_count() or 1
-    p = O.Params.from_config(cfg, nthreads=cores)
     fields, particles, currents, const, _ = O.init_plasma(p)
     e = particles['electrons']
     e.update(thermal_momenta(electron_mass(cfg), e['m'].shape))
-    rho_prev = np.zeros((p.N, p.N))
-    grid = (np.arange(p.N) - p.N // 2) * p.h
-    rho = -np.exp(-(grid[:, None] ** 2 + grid[None, :] ** 2) / (2 * 0.25))
+    beam = O.beam_sorted(O.beam_from_records(driver_beam(1_000_000, p.xi_step)))
+    ahead = beam['xi'] >= -start_step * p.xi_step
+    beam = {k: v[~ahead] for k, v in beam.items()}
     orig = O.compute_rhoj
 
     def serial_rhoj(pp, particles_, const_, nthreads=None):
@@ -209,22 +230,41 @@ def cpu_baseline(sample_steps: int) -> dict:
     O.compute_rhoj = serial_rhoj
     try:
         t0 = time.perf_counter()
-        for _ in range(sample_steps):
-            particles, fields, currents = O.step_dxi(p, particles, fields, currents, const,
-                                                     rho, rho_prev)
-        dt = time.perf_counter() - t0
+        O.step_dt(p, fields, particles, currents, const, -start_step * p.xi_step, beam,
+                  n_steps=steps)
+        return time.perf_counter() - t0
     finally:
         O.compute_rhoj = orig
+
+
+def cpu_baseline(sample_steps: int, one_core_steps: int) -> dict:
+    """The oracle (float64 numpy + C, this repository's CPU restatement of
+    the reference's Numba path) on the same workload: the xi loop with the
+    driver's layers (deposit, substeps, push) and the plasma step, pushes on
+    every host core (OpenMP, as the reference's Numba prange), deposit serial
+    (as weights.py:35), scipy.fftpack DST/DCT.  Timed over xi steps around
+    the driver's centre (xi = -3), on all cores and on one."""
+    from lcodepy_amd.config import Config
+    from oracle import lcode_oracle as O
+    cfg = Config(workload_config('fp64'))
+    cfg._adjust_config_values_3d()
+    cores = os.cpu_count() or 1
+    start = 140
+    dt_all = _oracle_sample(O.Params.from_config(cfg, nthreads=cores), cfg, sample_steps, start)
+    dt_one = _oracle_sample(O.Params.from_config(cfg, nthreads=1), cfg, one_core_steps, start)
     model = ''
     try:
         with open('/proc/cpuinfo') as f:
             model = next(l.split(':', 1)[1].strip() for l in f if l.startswith('model name'))
     except (OSError, StopIteration):
         pass
-    return {'value': sample_steps / dt, 'unit': 'xi-steps/s', 'cores': cores, 'kind': 'port',
-            'sample': f'{sample_steps} plasma xi steps of the same 513x513, 929,296-particle '
-                      f'float64 state (beam density fixed), deposit serial, pushes on {cores} '
-                      f'threads; {model}'}
+    return {'value': sample_steps / dt_all, 'unit': 'xi-steps/s', 'cores': cores, 'kind': 'port',
+            'one_core': {'value': one_core_steps / dt_one, 'unit': 'xi-steps/s', 'cores': 1,
+                         'steps': one_core_steps},
+            'sample': f'{sample_steps} xi steps ({one_core_steps} on one core) of the same '
+                      f'513x513, 929,296-particle float64 workload from xi step {start + 1} '
+                      f'(driver centre: beam layers deposited and moved), deposit serial, '
+                      f'pushes on {cores} threads; {model}'}
 
 
 def main() -> int:
@@ -237,9 +277,12 @@ def main() -> int:
                     help='xi steps with per-launch event timing after the timed steps '
                          '(default: the rest of the window, at most 91)')
     ap.add_argument('--cpu-sample-steps', type=int, default=20)
+    ap.add_argument('--cpu-one-core-steps', type=int, default=4)
     ap.add_argument('--no-cpu-baseline', action='store_true')
     ap.add_argument('--fp64-only', action='store_true')
-    ap.add_argument('--field-solver', default='matrix', choices=('matrix', 'rfft'))
+    ap.add_argument('--field-solver', default='auto', choices=('auto', 'matrix', 'rfft'),
+                    help="the library's field solve: 'auto' (default) times both on this GPU "
+                         'and takes the faster')
     ap.add_argument('--dump-ez', default=None,
                     help='write the on-axis Ez of every timed xi step of each form (npz)')
     ap.add_argument('--copy-calibration-mib', type=int, default=256,
@@ -276,8 +319,11 @@ def main() -> int:
         elapsed = time_steps(run, args.warmup, args.steps)
         ez = run.ez()
         kt = live_kernel_times(run, timing_steps)
-        run.solver.plasma_solver.ctx.raise_on_errors()
-        results[form] = dict(elapsed=elapsed, ez=ez, n=run.n_plasma, N=run.N, kt=kt)
+        ctx = run.solver.plasma_solver.ctx
+        ctx.raise_on_errors()
+        solve = {'field_solver': ctx.field_solver, 'fft_radix': ctx.fft_radix,
+                 'rfft_cap': ctx.rfft_shape[0], 'rfft_tpb': ctx.rfft_shape[1]}
+        results[form] = dict(elapsed=elapsed, ez=ez, n=run.n_plasma, N=run.N, kt=kt, solve=solve)
         print(f'[bench] {form}: {args.steps / elapsed:.1f} xi-steps/s, per launch '
               + ', '.join(f'{k} {v[0] * 1e6:.1f} us x{v[1]}' for k, v in kt.items()),
               file=sys.stderr, flush=True)
@@ -289,25 +335,41 @@ def main() -> int:
     value = args.steps / r['elapsed']
     N, n = r['N'], r['n']
     bulk = 4 if head == 'fp32-bulk' else 8
+    solver = r['solve']['field_solver']
     kt = {k: v[0] for k, v in r['kt'].items()}
-    per_step = {k: kt[k] * LAUNCHES_PER_STEP[k] for k in kt}
+    launches = {'push': LAUNCHES_PER_STEP['push'], 'deposit': LAUNCHES_PER_STEP['deposit'],
+                'solve': LAUNCHES_PER_STEP['solve'][solver]}
+    per_step = {k: kt[k] * launches[k] for k in kt}
     dominant = max(per_step, key=per_step.get)
 
     def entry(k):
-        if k == 'solve_gemm':
+        common = {'traffic': hbm_traffic(head, k), 'seconds_per_launch': kt[k],
+                  'seconds_per_xi_step': per_step[k]}
+        if k == 'solve' and solver == 'matrix':
             fl = gemm_flops(N)
             return {'bound': 'mfma', 'kernel': 'k_gemm_mfma (solve transform pass, 7 fields)',
                     'achieved': fl / kt[k] / 1e12, 'peak': MFMA_PEAK[head] / 1e12,
-                    'unit': 'TFLOP/s', 'frac': fl / kt[k] / MFMA_PEAK[head],
-                    'traffic': hbm_traffic(head, k), 'seconds_per_launch': kt[k],
-                    'seconds_per_xi_step': per_step[k]}
-        nbytes = push_bytes(n, N, bulk) if k == 'push' else deposit_bytes(n, N, bulk)
-        name = 'k_push (smart push, 3 per xi step)' if k == 'push' else \
-            'k_deposit_gather (plasma deposit, 3 per xi step)'
+                    'unit': 'TFLOP/s', 'frac': fl / kt[k] / MFMA_PEAK[head], **common}
+        if k == 'solve':
+            nbytes = solve_bytes(N, head) / 3
+            name = ('k_rf_rows_fwd / k_rf_cols / k_rf_rows_inv (LDS-FFT solve, 3 launches '
+                    'per solve, 68 N^2 words per solve)')
+        elif k == 'push':
+            nbytes = push_bytes(n, N, bulk)
+            name = 'k_push (smart push, 3 per xi step)'
+        else:
+            nbytes = deposit_bytes(n, N, bulk)
+            name = 'deposit: cell sort + k_deposit_gather (3 per xi step)'
         return {'bound': 'hbm', 'kernel': name, 'achieved': nbytes / kt[k] / 1e9,
                 'peak': HBM_PEAK / 1e9, 'unit': 'GB/s', 'frac': nbytes / kt[k] / HBM_PEAK,
-                'traffic': hbm_traffic(head, k), 'seconds_per_launch': kt[k],
-                'seconds_per_xi_step': per_step[k]}
+                **common}
+
+    b_step = step_bytes(n, N, head)
+    roofline_step = {'bound': 'hbm', 'bytes_per_xi_step': b_step,
+                     'achieved': b_step * value / 1e9, 'peak': HBM_PEAK / 1e9, 'unit': 'GB/s',
+                     'frac': b_step * value / HBM_PEAK,
+                     'model': 'B_step = Np b_p + 166 N^2 b_g (46 words per particle, 166 per '
+                              'grid point)'}
 
     line = {
         'metric': METRIC, 'value': value, 'unit': 'xi-steps/s', 'n_gpus': 1,
@@ -321,12 +383,17 @@ def main() -> int:
                                'xi_step 0.02 (1001 xi steps), 1e6-particle driver n_b=1 '
                                'sigma_r=0.5 sigma_xi=1',
                    'precision': head, 'N': N, 'plasma_particles': n,
-                   'beam_particles': args.beam_particles},
+                   'beam_particles': args.beam_particles,
+                   'thermal_momenta': 'normalised: p / (m c) ~ N(0, 1e-3) per component, '
+                                      'stored p = m u (read as stored momenta the plasma '
+                                      'is thermally chaotic and no precision tracks it)'},
         'particle_pushes_per_s': 3 * n * value,
         'kernel_avg_s': kt,
         'kernel_launches_timed': {k: v[1] for k, v in r['kt'].items()},
         'roofline': entry(dominant),
         'roofline_all': {k: entry(k) for k in kt},
+        'roofline_step': roofline_step,
+        'field_solve': r['solve'],
     }
     if 'fp64' in results and head != 'fp64':
         r64 = results['fp64']
@@ -345,7 +412,7 @@ def main() -> int:
         np.savez(args.dump_ez, **{form.replace('-', '_'): res['ez'] for form, res in results.items()})
     if not args.no_cpu_baseline:
         print('[bench] cpu baseline (oracle)', file=sys.stderr, flush=True)
-        line['cpu_baseline'] = cpu_baseline(args.cpu_sample_steps)
+        line['cpu_baseline'] = cpu_baseline(args.cpu_sample_steps, args.cpu_one_core_steps)
     print(json.dumps(line), flush=True)
     return 0
 

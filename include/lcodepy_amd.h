@@ -99,6 +99,19 @@ typedef struct lc3d_params {
      * taking 2(N-1) values of a buffer (0 = as many as fit) */
     int32_t rfft_cap;
     int32_t rfft_tpb;
+    /* declustering noise filter of the electrons (plasma3d/noise_filter.py:
+     * 82-238; declustering-enabled): side of their (side, side) lattice, 0 =
+     * off; the circular Gaussian smoothing as float64 taps for lattice
+     * offsets tap0 .. tap0 + ntaps - 1 (host array, copied); the
+     * declustering-force, the damping (read by the reference from the key
+     * 'damping-declustering') and declustering-limit */
+    int32_t declustering_side;
+    int32_t declustering_ntaps;
+    int32_t declustering_tap0;
+    const double *declustering_taps;
+    double declustering_force;
+    double declustering_damping;
+    double declustering_limit;
 } lc3d_params;
 
 /* Allocates every workspace (cell bins, spectral scratch, transform
@@ -112,18 +125,40 @@ int lc3d_precision(const lc3d_ctx *ctx);
  * (synchronises `stream`). */
 int lc3d_error_flags(lc3d_ctx *ctx, uint32_t *flags, void *stream);
 
-/* Measurement, with no counterpart in the reference: device time per launch
- * of the three heaviest kernels, from HIP events recorded on the launch
- * stream around them.  enable != 0 starts it (and clears the totals); 0 stops
- * it.  lc3d_kernel_times waits for the recorded events and gives, for
- * [push, deposit gather, solve transform GEMM], the average seconds per
- * launch and the launch counts since timing was enabled. */
+/* Measurement, with no counterpart in the reference: device time of the
+ * three heaviest stages, from HIP events recorded on the launch stream around
+ * them.  enable != 0 starts it (and clears the totals); 0 stops it.
+ * lc3d_kernel_times waits for the recorded events and gives, for [smart
+ * push, whole deposit (cell sort and gather, one launch each), solve
+ * transform launch (a GEMM of the matrix solve, one of the three FFT kernels
+ * of the RFFT solve)], the average seconds per launch and the launch counts
+ * since timing was enabled. */
 #define LC3D_KC_PUSH 0
-#define LC3D_KC_DEPOSIT_GATHER 1
-#define LC3D_KC_SOLVE_GEMM 2
+#define LC3D_KC_DEPOSIT 1
+#define LC3D_KC_SOLVE 2
 int lc3d_kernel_timing(lc3d_ctx *ctx, int32_t enable);
 int lc3d_kernel_times(lc3d_ctx *ctx, double seconds[3], int64_t launches[3]);
 
+/* ---- field-solve autotuner -------------------------------------------------
+ * No counterpart in the reference (its solve is fixed: scipy.fftpack on the
+ * CPU, padded cuFFT on the GPU, plasma3d/fields.py:12-109). */
+typedef struct lc3d_solve_choice {
+    int32_t field_solver;        /* LC3D_SOLVER_MATRIX or LC3D_SOLVER_RFFT */
+    int32_t fft_nstages;         /* RFFT radix plan (length 2(N-1)) */
+    int32_t fft_radix[12];
+    int32_t rfft_cap;            /* RFFT LDS size and transforms per block */
+    int32_t rfft_tpb;
+    double seconds;              /* device time of one lc3d_compute_fields (< 0: refused) */
+} lc3d_solve_choice;
+/* Times lc3d_compute_fields for the geometry and precision of *params on
+ * synthetic inputs, `repeats` times after two warm-up calls, for the matrix
+ * solve and the RFFT solve over its radix plans x LDS sizes x transforms per
+ * block; writes the fastest to *best and, when table != NULL, the first
+ * table_cap candidates in the order tried (*n_tried = how many were timed).
+ * Creates its own contexts and inputs and synchronises the device: a set-up
+ * call, not for the xi loop.  Copy *best's fields into lc3d_params to use it. */
+int lc3d_autotune_solve(const lc3d_params *params, int32_t repeats, lc3d_solve_choice *best,
+                        lc3d_solve_choice *table, int32_t table_cap, int32_t *n_tried);
 /* ---- plasma particles ------------------------------------------------------ */
 typedef struct lc3d_fields6 {   /* the six fields a push reads */
     const void *Ex, *Ey, *Ez, *Bx, *By, *Bz;
@@ -140,6 +175,10 @@ typedef struct lc3d_sort {
     void *px, *py, *q;                         /* bulk type */
     double *pz;
     int32_t column;                    /* current column of this sort */
+    /* chaotic displacements of the noise filter (float64, n each): layer k
+     * read, layer k+1 written; NULL for a sort the filter does not touch */
+    const double *dx_chaotic_prev, *dy_chaotic_prev;
+    double *dx_chaotic, *dy_chaotic;
 } lc3d_sort;
 
 /* x_offt += px / (gamma - pz) * xi_step where m != 0 (in place). */
@@ -159,6 +198,13 @@ int lc3d_move_smart(lc3d_ctx *ctx, const lc3d_sort *sort, const lc3d_fields6 *fi
 int lc3d_deposit(lc3d_ctx *ctx, const lc3d_sort *sort, void *ro, void *jx, void *jy, void *jz,
                  const void *bg_ro, void *ro1, void *jx1, void *jy1, void *jz1, void *stream);
 
+/* noise_filter(particles, particles_prev), plasma3d/noise_filter.py:102-238,
+ * of one sort whose n is the context's declustering_side squared: kicks px,
+ * py in place from x_offt, y_offt and writes dx_chaotic, dy_chaotic (the
+ * sort's chaotic pointers must be set).  lc3d_plasma_step applies it to
+ * sorts[0] after the last push when the context has the filter and sorts[0]
+ * has the chaotic pointers (plasma3d/solver.py:69-73). */
+int lc3d_noise_filter(lc3d_ctx *ctx, const lc3d_sort *sort, void *stream);
 /* ---- fields ------------------------------------------------------------------- */
 typedef struct lc3d_fields {   /* Ex Ey Ez Bx By Bz Phi, each (N, N) */
     void *Ex, *Ey, *Ez, *Bx, *By, *Bz, *Phi;

@@ -47,7 +47,14 @@ class Params(ctypes.Structure):
                 ('eigenvalues', ctypes.POINTER(ctypes.c_double)),
                 ('field_solver', ctypes.c_int32),
                 ('rfft_cap', ctypes.c_int32),
-                ('rfft_tpb', ctypes.c_int32)]
+                ('rfft_tpb', ctypes.c_int32),
+                ('declustering_side', ctypes.c_int32),
+                ('declustering_ntaps', ctypes.c_int32),
+                ('declustering_tap0', ctypes.c_int32),
+                ('declustering_taps', ctypes.POINTER(ctypes.c_double)),
+                ('declustering_force', ctypes.c_double),
+                ('declustering_damping', ctypes.c_double),
+                ('declustering_limit', ctypes.c_double)]
 
 
 class Fields6(ctypes.Structure):
@@ -60,6 +67,15 @@ class Fields(ctypes.Structure):
 
 class Currents(ctypes.Structure):
     _fields_ = [(k, _vp) for k in ('ro', 'jx', 'jy', 'jz')]
+
+
+class SolveChoice(ctypes.Structure):
+    _fields_ = [('field_solver', ctypes.c_int32),
+                ('fft_nstages', ctypes.c_int32),
+                ('fft_radix', ctypes.c_int32 * 12),
+                ('rfft_cap', ctypes.c_int32),
+                ('rfft_tpb', ctypes.c_int32),
+                ('seconds', ctypes.c_double)]
 
 
 PROBE_MAX_SOURCES = 16
@@ -78,14 +94,17 @@ class Sort(ctypes.Structure):
                 ('px_prev', _vp), ('py_prev', _vp), ('q_prev', _vp), ('pz_prev', _vp),
                 ('x_offt', _vp), ('y_offt', _vp),
                 ('px', _vp), ('py', _vp), ('q', _vp), ('pz', _vp),
-                ('column', ctypes.c_int32)]
+                ('column', ctypes.c_int32),
+                ('dx_chaotic_prev', _vp), ('dy_chaotic_prev', _vp),
+                ('dx_chaotic', _vp), ('dy_chaotic', _vp)]
 
 
 SYMBOLS = ('lc3d_create', 'lc3d_destroy', 'lc3d_last_error', 'lc3d_precision',
            'lc3d_error_flags', 'lc3d_kernel_timing', 'lc3d_kernel_times', 'lc3d_move_wo_fields', 'lc3d_move_smart', 'lc3d_deposit',
            'lc3d_compute_fields', 'lc3d_plasma_step', 'lc3d_deposit_beam',
            'lc3d_beam_substep', 'lc3d_move_beam', 'lc3d_energy_flux', 'lc3d_probe_sample',
-           'lc3d_abs_max', 'lc3d_energy_flux_noalloc')
+           'lc3d_abs_max', 'lc3d_energy_flux_noalloc', 'lc3d_autotune_solve',
+           'lc3d_noise_filter')
 
 _LIB = None
 _LOCK = threading.Lock()
@@ -121,6 +140,9 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
     L.lc3d_energy_flux.argtypes = [_vp, P(Fields6), i64, _vp, _vp, _vp, _vp, _vp, _vp]
     L.lc3d_energy_flux_noalloc.argtypes = [i32, dbl, i32, P(Fields6), i64, _vp, _vp, _vp, _vp,
                                            _vp, _vp]
+    L.lc3d_autotune_solve.argtypes = [P(Params), i32, P(SolveChoice), P(SolveChoice), i32,
+                                      P(i32)]
+    L.lc3d_noise_filter.argtypes = [_vp, P(Sort), _vp]
     L.lc3d_probe_sample.argtypes = [_vp, P(ProbeSources), i32, _vp, _vp, _vp, _vp]
     L.lc3d_abs_max.argtypes = [_vp, _vp, i64, i64, i64, i64, i64, i64, _vp, _vp]
     L.lc3d_move_beam.argtypes = [_vp, i64, i64, _vp, _vp, P(Fields6), P(Fields6), _vp, _vp,
@@ -163,7 +185,7 @@ class Context:
                  subtraction_coeff: float, lost_radius: float, precision: str,
                  max_particles: int, eigenvalues: np.ndarray | None = None,
                  fft_radix: tuple | None = None, field_solver: str = 'matrix',
-                 rfft_cap: int = 0, rfft_tpb: int = 0):
+                 rfft_cap: int = 0, rfft_tpb: int = 0, declustering: dict | None = None):
         import torch
         if not torch.cuda.is_available():
             raise RuntimeError('lcodepy_amd needs a ROCm GPU (torch.cuda.is_available() is False)')
@@ -172,6 +194,9 @@ class Context:
         self.h = float(h)
         self.precision = precision
         self.max_particles = int(max_particles)
+        self.field_solver = field_solver
+        self.fft_radix = tuple(fft_radix) if fft_radix else None
+        self.rfft_shape = (int(rfft_cap), int(rfft_tpb))
         p = Params()
         p.grid_steps = N
         p.grid_step_size = h
@@ -186,6 +211,17 @@ class Context:
         p.field_solver = SOLVERS[field_solver]
         p.rfft_cap = int(rfft_cap)
         p.rfft_tpb = int(rfft_tpb)
+        self.declustering_side = 0
+        self._dc_taps = None
+        if declustering is not None:
+            self._dc_taps = np.ascontiguousarray(declustering['taps'], dtype=np.float64)
+            p.declustering_side = self.declustering_side = int(declustering['side'])
+            p.declustering_ntaps = len(self._dc_taps)
+            p.declustering_tap0 = int(declustering['tap0'])
+            p.declustering_taps = self._dc_taps.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+            p.declustering_force = float(declustering['force'])
+            p.declustering_damping = float(declustering['damping'])
+            p.declustering_limit = float(declustering['limit'])
         if fft_radix:
             p.fft_nstages = len(fft_radix)
             for i, r in enumerate(fft_radix):
@@ -223,11 +259,12 @@ class Context:
                    'lc3d_error_flags')
         return int(v.value)
 
-    KERNEL_CLASSES = ('push', 'deposit_gather', 'solve_gemm')
+    KERNEL_CLASSES = ('push', 'deposit', 'solve')
 
     def kernel_timing(self, enable: bool) -> None:
-        """Start (and clear) or stop the per-launch event timing of the
-        push, the deposit gather and the solve's transform GEMMs."""
+        """Start (and clear) or stop the event timing of the smart push, the
+        whole deposit (cell sort and gather) and the solve's transform
+        launches."""
         self.check(self.lib.lc3d_kernel_timing(self.handle, int(bool(enable))),
                    'lc3d_kernel_timing')
 

@@ -197,9 +197,13 @@ class Config:
                 f"use one of {HIP_PU_TYPES} (this package implements the HIP path only)")
 
     def field_solver(self) -> str:
-        s = self.get('field-solver', 'matrix').lower()
-        if s not in ('matrix', 'fft', 'rfft'):
-            raise ValueError(f"field-solver must be 'matrix', 'fft' or 'rfft', got {s!r}")
+        """'auto' (the default: the faster of the two GPU solves for this grid
+        and precision, measured once per process by lcodepy_amd.autotune),
+        'rfft' (paired LDS FFTs), 'matrix' (transform-matrix products on the
+        matrix cores) or 'fft' (refused by the GPU library)."""
+        s = self.get('field-solver', 'auto').lower()
+        if s not in ('auto', 'matrix', 'fft', 'rfft'):
+            raise ValueError(f"field-solver must be 'auto', 'matrix', 'fft' or 'rfft', got {s!r}")
         return s
 
     def precision(self) -> str:

@@ -47,6 +47,9 @@ struct lc3d_ctx {
     int rf_cap = 0, rf_tpb = 0;                              // RFFT launch shape
     void *tdst = nullptr, *tdct = nullptr, *tdct_t = nullptr;   // transform matrices
     double *lam = nullptr;
+    // declustering noise filter
+    int dc_side = 0, dc_ntaps = 0, dc_tap0 = 0;
+    double *dc_taps = nullptr, *dc_scratch = nullptr;
     KernelTimer timer;
     std::string last_error;
 };
@@ -287,6 +290,30 @@ bool sort_ok(const lc3d_ctx *c, const lc3d_sort *s)
            (s->column == 0 || s->column == 1);
 }
 
+bool declust_ok(const lc3d_ctx *c, const lc3d_sort *s)
+{
+    return c->dc_side > 0 && s->n == (int64_t)c->dc_side * c->dc_side && s->dx_chaotic_prev &&
+           s->dy_chaotic_prev && s->dx_chaotic && s->dy_chaotic;
+}
+
+int noise_filter_sort(lc3d_ctx *c, const lc3d_sort *s, hipStream_t st)
+{
+    const DeclustGeom g{c->dc_side, c->dc_ntaps, c->dc_tap0, c->dc_taps,
+                        c->prm.xi_step_size, c->prm.declustering_force,
+                        c->prm.declustering_damping, c->prm.declustering_limit, c->dc_scratch};
+    hipError_t e;
+    if (fp32(c))
+        e = launch_noise_filter<float>(g, s->x_offt, s->y_offt, static_cast<float *>(s->px),
+                                       static_cast<float *>(s->py), s->dx_chaotic_prev,
+                                       s->dy_chaotic_prev, s->dx_chaotic, s->dy_chaotic, st);
+    else
+        e = launch_noise_filter<double>(g, s->x_offt, s->y_offt, static_cast<double *>(s->px),
+                                        static_cast<double *>(s->py), s->dx_chaotic_prev,
+                                        s->dy_chaotic_prev, s->dx_chaotic, s->dy_chaotic, st);
+    if (e != hipSuccess) return hip_fail(c, e, "noise_filter");
+    return LC3D_OK;
+}
+
 bool fields_ok(const lc3d_fields *f, bool phi)
 {
     return f && f->Ex && f->Ey && f->Ez && f->Bx && f->By && f->Bz && (!phi || f->Phi);
@@ -440,6 +467,18 @@ int lc3d_create(const lc3d_params *p, lc3d_ctx **out)
             return LC3D_EINVAL;
         }
     }
+    if (p->declustering_side != 0) {
+        const int side = p->declustering_side, nt = p->declustering_ntaps;
+        if (side < 3 || side > 4096 || nt < 1 || nt > side || !p->declustering_taps ||
+            p->declustering_limit <= 0) {
+            delete c;
+            return LC3D_EINVAL;
+        }
+        c->dc_side = side;
+        c->dc_ntaps = nt;
+        c->dc_tap0 = p->declustering_tap0;
+    }
+    c->prm.declustering_taps = nullptr;
     (void)hipGetDevice(&c->device);
     const size_t nn = (size_t)N * N;
     const size_t nblocks = (nn + 1 + 2047) / 2048;
@@ -464,6 +503,10 @@ int lc3d_create(const lc3d_params *p, lc3d_ctx **out)
               hipMalloc(&c->diag, 2 * DIAG_BLOCKS * sizeof(double)) == hipSuccess &&
               hipMalloc(&c->lam, N * sizeof(double)) == hipSuccess;
     if (ok && (fft || rfft)) ok = hipMalloc(&c->tw, 2 * (size_t)M * 2 * tg) == hipSuccess;
+    if (ok && c->dc_side > 0)
+        ok = hipMalloc(&c->dc_taps, c->dc_ntaps * sizeof(double)) == hipSuccess &&
+             hipMalloc(&c->dc_scratch, 4 * (size_t)c->dc_side * c->dc_side * sizeof(double)) ==
+                 hipSuccess;
     if (ok && !fft && !rfft)
         ok = hipMalloc(&c->work2, 7 * nn * tg) == hipSuccess &&
              hipMalloc(&c->tdst, nn * tg) == hipSuccess &&
@@ -521,6 +564,9 @@ int lc3d_create(const lc3d_params *p, lc3d_ctx **out)
         if (e == hipSuccess) e = upload(c->tdct_t, Ct);
     }
     if (e == hipSuccess) e = hipMemcpy(c->lam, lam.data(), N * sizeof(double), hipMemcpyHostToDevice);
+    if (e == hipSuccess && c->dc_side > 0)
+        e = hipMemcpy(c->dc_taps, p->declustering_taps, c->dc_ntaps * sizeof(double),
+                      hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemset(c->count, 0, (nn + 1) * sizeof(int));
     if (e == hipSuccess) e = hipMemset(c->err, 0, sizeof(unsigned));
     if (e == hipSuccess) e = hipMemset(c->nbig, 0, sizeof(int));
@@ -563,6 +609,8 @@ void lc3d_destroy(lc3d_ctx *c)
     (void)hipFree(c->tdct);
     (void)hipFree(c->tdct_t);
     (void)hipFree(c->lam);
+    (void)hipFree(c->dc_taps);
+    (void)hipFree(c->dc_scratch);
     delete c;
 }
 
@@ -722,12 +770,26 @@ int lc3d_plasma_step(lc3d_ctx *c, int32_t n_sorts, const lc3d_sort *sorts,
     }
     // solver.py:59-62: solve 2 with fields_half as predictor.
     if ((r = solve(c, half, f32, prev, rbF, rbP, cp, cf, full, half, true, st))) return r;
-    // solver.py:64-76: push 3 in the new fields_half, deposit 3.
+    // solver.py:64-76: push 3 in the new fields_half, the electrons' noise
+    // filter when on (it changes px, py, so that sort is binned by its
+    // deposit instead of by the push), deposit 3.
     for (int s = 0; s < n_sorts; ++s) {
-        if ((r = push_sort(c, &sorts[s], fhalf, f32, false, true, st))) return r;
-        if ((r = deposit_sort(c, &sorts[s], true, cf, bg_ro, single, st))) return r;
+        const bool filter = s == 0 && declust_ok(c, &sorts[s]);
+        if ((r = push_sort(c, &sorts[s], fhalf, f32, false, !filter, st))) return r;
+        if (filter && (r = noise_filter_sort(c, &sorts[s], st))) return r;
+        if ((r = deposit_sort(c, &sorts[s], !filter, cf, bg_ro, single, st))) return r;
     }
     return LC3D_OK;
+}
+
+int lc3d_noise_filter(lc3d_ctx *c, const lc3d_sort *s, void *stream)
+{
+    if (!c || !s || !sort_ok(c, s))
+        return fail(c, LC3D_EINVAL, "noise_filter: bad arguments");
+    if (!declust_ok(c, s))
+        return fail(c, LC3D_EINVAL, "noise_filter: the context has no filter for a lattice of "
+                                    "this size, or the sort lacks its chaotic arrays");
+    return noise_filter_sort(c, s, static_cast<hipStream_t>(stream));
 }
 
 int lc3d_deposit_beam(lc3d_ctx *c, int64_t n, int64_t layer, const double *x, const double *y,

@@ -507,6 +507,8 @@ hipError_t launch_deposit(const PushGeom &g, const SortArrays<TP> &a, bool use_f
     const TP *q = use_full ? a.q : a.q_prev;
     const int pblocks = (int)((g.n + 255) / 256);
     const int rec_words = (int)(sizeof(DepRec<TP>) / 16);
+    hipError_t e;
+    if (timer && (e = timer->begin(KC_DEPOSIT, 1, s)) != hipSuccess) return e;
     if (g.n > 0) {
         if (!binned)
             k_bin<TP><<<pblocks, 256, 0, s>>>(g.n, g.h, N, a.x_init, a.y_init, xo, yo, px, py, pz,
@@ -529,8 +531,6 @@ hipError_t launch_deposit(const PushGeom &g, const SortArrays<TP> &a, bool use_f
     const int tpr = (N + DEP_TILE - 1) / DEP_TILE;
     const int tiles = tpr * tpr;
     constexpr bool kahan = sizeof(TP) == 4;
-    hipError_t e;
-    if (timer && (e = timer->begin(KC_DEPOSIT_GATHER, 1, s)) != hipSuccess) return e;
     k_deposit_gather<TP, TG, kahan><<<tiles, 256, 0, s>>>(N, g.h, bs.start, bs, ro, jx, jy, jz,
                                                           bg_ro, ro1, jx1, jy1, jz1);
     if ((e = hipGetLastError()) != hipSuccess) return e;

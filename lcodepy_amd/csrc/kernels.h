@@ -79,10 +79,13 @@ inline size_t deposit_record_bytes(bool fp32_bulk)
     return fp32_bulk ? sizeof(DepRec<float>) : sizeof(DepRec<double>);
 }
 
-// Device time per launch of the heaviest kernels, measured with HIP events
-// recorded on the launch stream around them (off unless enabled on the
-// context).  An interval may span several back-to-back launches of a class.
-enum KernelClass { KC_PUSH = 0, KC_DEPOSIT_GATHER = 1, KC_SOLVE_GEMM = 2, KC_COUNT = 3 };
+// Device time of the heaviest stages, measured with HIP events recorded on
+// the launch stream around them (off unless enabled on the context): one
+// smart push, one whole deposit (cell sort and gather, counted as one
+// launch) and the solve's transform launches (the GEMMs of the matrix solve,
+// the three FFT kernels of the LDS-FFT solve).  An interval may span several
+// back-to-back launches of a class.
+enum KernelClass { KC_PUSH = 0, KC_DEPOSIT = 1, KC_SOLVE = 2, KC_COUNT = 3 };
 
 struct KernelTimer {
     static constexpr int CAP = 64;   // intervals recorded before they are read back
@@ -224,6 +227,23 @@ template <typename TG, typename TPRED, typename TFULL, typename THALF>
 hipError_t launch_compute_fields_rfft(const RfftGeom &g, const SolveIn<TG, TPRED> &in,
                                       const SolveOut<TFULL, THALF> &out, bool write_full,
                                       hipStream_t s, KernelTimer *timer = nullptr);
+
+// ---- declustering noise filter (plasma3d/noise_filter.py) -------------------------
+struct DeclustGeom {
+    int side;               // lattice side n of the sort (n * n particles)
+    int ntaps, tap0;        // circular Gaussian taps for offsets tap0 .. tap0 + ntaps - 1
+    const double *taps;
+    double xi_step, force, damping, limit;
+    double *scratch;        // 4 n * n float64
+};
+
+// noise_filter(particles, particles_prev) of one sort: px, py kicked in place
+// (bulk type TP), dx_chaotic, dy_chaotic written from x_offt, y_offt and the
+// previous chaotic displacements.
+template <typename TP>
+hipError_t launch_noise_filter(const DeclustGeom &g, const double *x_offt, const double *y_offt,
+                               TP *px, TP *py, const double *dx_prev, const double *dy_prev,
+                               double *dx_chaotic, double *dy_chaotic, hipStream_t s);
 
 // ---- diagnostics ---------------------------------------------------------------
 // diagnostics/FXi/Strategy3D.py:6-25: total energy flux of a plasma layer into

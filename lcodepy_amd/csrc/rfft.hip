@@ -14,8 +14,10 @@
 // field f with right-hand side R is Ty Tx (Lambda .* (Tx Ty R)), Lambda the
 // solver matrix of initialization.py:15-67 divided by L^2 (solver_factor).
 //
-// Kernels (all seven fields in one launch, blockIdx.y = field; rows / columns
-// paired (2p, 2p + 1), the last pair of the odd N holding one):
+// Kernels (all seven fields in one launch; rows / columns paired (2p, 2p + 1),
+// the last pair of the odd N holding one; a 1-D grid whose blocks are dealt
+// to the XCDs so that blocks reading the same rows or cache lines share an
+// L2, see rf_block):
 //   k_rf_rows_fwd  right-hand sides of two rows (evaluated here from the
 //                  currents), transform along y, spectral rows into work;
 //   k_rf_cols      two spectral columns: transform along x, times Lambda,
@@ -53,6 +55,21 @@ template <typename T> struct Cx {
     }
     static __device__ __forceinline__ C negi(C a) { return make(a.y, -a.x); }   // a * (-i)
     static __device__ __forceinline__ C neg(C a) { return make(-a.x, -a.y); }
+};
+
+// x / d and x mod d for 0 <= x < 2^22, d >= 1, by one float multiply: the
+// true quotient (x + 1/2) / d is at least 1/(2d) from an integer, and the
+// float product's relative error (< 2^-23) stays below that for such x.
+struct RfDiv {
+    int d;
+    float inv;
+    __device__ __forceinline__ explicit RfDiv(int d_) : d(d_), inv(1.0f / (float)d_) {}
+    __device__ __forceinline__ int div(int x) const
+    {
+        const int q = (int)(((float)x + 0.5f) * inv);
+        RF_CHECK(x >= 0 && x < (1 << 22) && q == x / d);
+        return q;
+    }
 };
 
 // Forward 4-point DFT in place.
@@ -192,8 +209,9 @@ __device__ __forceinline__ void rf_stage(const cplx_t<T> *src, cplx_t<T> *dst,
     using C = cplx_t<T>;
     using X = Cx<T>;
     const int LR = L / R, total = ntr * LR, tws = L / (Ns * R);
+    const RfDiv dLR(LR), dNs(Ns);
     for (int g = threadIdx.x; g < total; g += RF_T) {
-        const int tr = g / LR, b = g - tr * LR, k = b % Ns;
+        const int tr = dLR.div(g), b = g - tr * LR, k = b - dNs.div(b) * Ns;
         const C *in = src + tr * L + b;
         RF_CHECK(tr < ntr && b + (R - 1) * LR < L);
         C v[R];
@@ -252,6 +270,28 @@ __device__ __forceinline__ int rf_block_transforms(int tpb, int pairs, int bx)
     return left < tpb ? left : tpb;
 }
 
+// Field and pair group of this block.  The hardware deals consecutive block
+// ids round-robin over the 8 XCDs; xcd_swizzle turns that into contiguous
+// runs of logical ids per XCD, in launch order.  With fields_fastest the
+// seven fields of a row group are consecutive (the row passes: they read the
+// same current rows), otherwise the groups of a field are (the column pass:
+// neighbouring column groups share cache lines).
+__device__ __forceinline__ void rf_block(int groups, bool fields_fastest, int &f, int &group)
+{
+    const int total = 7 * groups;
+    const int l = xcd_swizzle((int)blockIdx.x, total);
+    if (fields_fastest) {
+        group = l / 7;
+        f = l - 7 * group;
+    } else {
+        f = l / groups;
+        group = l - f * groups;
+    }
+    RF_CHECK(f >= 0 && f < 7 && group >= 0 && group < groups);
+}
+
+
+
 // ---- rows, forward: right-hand sides and the transform along y ---------------
 // CAP: complex values of the block's LDS, two buffers of CAP / 2.
 template <typename TG, typename TPRED, int CAP>
@@ -260,15 +300,18 @@ __global__ __launch_bounds__(RF_T) void k_rf_rows_fwd(RfftGeom g, RhsEval<TG, TP
     using C = cplx_t<TG>;
     using X = Cx<TG>;
     __shared__ __align__(16) C lds[CAP];
-    const int f = blockIdx.y, N = g.N, M = N - 1, L = 2 * M, per = M + 1;
-    const int p0 = blockIdx.x * g.tpb;
-    const int ntr = rf_block_transforms(g.tpb, (N + 1) / 2, blockIdx.x);
+    const int N = g.N, M = N - 1, L = 2 * M, per = M + 1, groups = (N + 1) / 2;
+    int f, grp;
+    rf_block((groups + g.tpb - 1) / g.tpb, true, f, grp);
+    const int p0 = grp * g.tpb;
+    const int ntr = rf_block_transforms(g.tpb, groups, grp);
     const bool ydst = dst_along_y(f);
     const C *tw = static_cast<const C *>(g.tw);
     TG *work = static_cast<TG *>(g.work) + (size_t)f * N * N;
+    const RfDiv dper(per);
     RF_CHECK(ntr >= 1 && 2 * ntr * L <= CAP);
     for (int it = threadIdx.x; it < ntr * per; it += RF_T) {
-        const int tr = it / per, j = it - tr * per;
+        const int tr = dper.div(it), j = it - tr * per;
         const int ia = 2 * (p0 + tr), ib = ia + 1;
         RF_CHECK(ia < N && j <= M);
         const TG a = in_rhs_domain(f, ia, j, N) ? rhs(f, ia, j) : TG(0);
@@ -280,7 +323,7 @@ __global__ __launch_bounds__(RF_T) void k_rf_rows_fwd(RfftGeom g, RhsEval<TG, TP
     __syncthreads();
     const C *z = rf_fft<TG>(lds, lds + CAP / 2, tw, g.stages, L, ntr);
     for (int it = threadIdx.x; it < ntr * per; it += RF_T) {
-        const int tr = it / per, k = it - tr * per;
+        const int tr = dper.div(it), k = it - tr * per;
         const int ia = 2 * (p0 + tr), ib = ia + 1;
         const C v = z[tr * L + k];
         const bool zero = ydst && (k == 0 || k == M);
@@ -298,10 +341,13 @@ __global__ __launch_bounds__(RF_T) void k_rf_cols(RfftGeom g)
     using C = cplx_t<TG>;
     using X = Cx<TG>;
     __shared__ __align__(16) C lds[CAP];
-    const int f = blockIdx.y, N = g.N, M = N - 1, L = 2 * M, per = M + 1;
-    const int p0 = blockIdx.x * g.tpb;
-    const int ntr = rf_block_transforms(g.tpb, (N + 1) / 2, blockIdx.x);
+    const int N = g.N, M = N - 1, L = 2 * M, per = M + 1, groups = (N + 1) / 2;
+    int f, grp;
+    rf_block((groups + g.tpb - 1) / g.tpb, false, f, grp);
+    const int p0 = grp * g.tpb;
+    const int ntr = rf_block_transforms(g.tpb, groups, grp);
     const int j0 = 2 * p0, nc = 2 * ntr;   // columns j0 .. j0 + nc - 1 (the last may be N)
+    const RfDiv dper(per), dnc(nc);
     const bool xdst = dst_along_x(f);
     const C *tw = static_cast<const C *>(g.tw);
     TG *work = static_cast<TG *>(g.work) + (size_t)f * N * N;
@@ -310,7 +356,7 @@ __global__ __launch_bounds__(RF_T) void k_rf_cols(RfftGeom g)
     RF_CHECK(ntr >= 1 && 2 * ntr * L <= CAP && j0 + nc <= N + 1);
     // Consecutive threads take consecutive columns of one row.
     for (int it = threadIdx.x; it < per * nc; it += RF_T) {
-        const int i = it / nc, c = it - i * nc;
+        const int i = dnc.div(it), c = it - i * nc;
         const int j = j0 + c, tr = c >> 1, part = c & 1;
         TG x = j < N ? work[(size_t)i * N + j] : TG(0);
         if (xdst && (i == 0 || i == M)) x = TG(0);
@@ -324,7 +370,7 @@ __global__ __launch_bounds__(RF_T) void k_rf_cols(RfftGeom g)
     // extension the inverse transform along x starts from.
     const ScaleGeom sg{g.lam, g.c, (double)L * (double)L};
     for (int it = threadIdx.x; it < ntr * per; it += RF_T) {
-        const int tr = it / per, k = it - tr * per;
+        const int tr = dper.div(it), k = it - tr * per;
         const int ja = j0 + 2 * tr, jb = ja + 1;
         const C v = z[tr * L + k];
         const TG ya = xdst ? -v.y : v.x;
@@ -339,7 +385,7 @@ __global__ __launch_bounds__(RF_T) void k_rf_cols(RfftGeom g)
     __syncthreads();
     z = rf_fft<TG>(e, z, tw, g.stages, L, ntr);
     for (int it = threadIdx.x; it < per * nc; it += RF_T) {
-        const int i = it / nc, c = it - i * nc;
+        const int i = dnc.div(it), c = it - i * nc;
         const int j = j0 + c, tr = c >> 1, part = c & 1;
         if (j < N) {
             const C v = z[tr * L + i];
@@ -359,15 +405,18 @@ __global__ __launch_bounds__(RF_T) void k_rf_rows_inv(RfftGeom g, SolveOut<TFULL
     using C = cplx_t<TG>;
     using X = Cx<TG>;
     __shared__ __align__(16) C lds[CAP];
-    const int f = blockIdx.y, N = g.N, M = N - 1, L = 2 * M, per = M + 1;
-    const int p0 = blockIdx.x * g.tpb;
-    const int ntr = rf_block_transforms(g.tpb, (N + 1) / 2, blockIdx.x);
+    const int N = g.N, M = N - 1, L = 2 * M, per = M + 1, groups = (N + 1) / 2;
+    int f, grp;
+    rf_block((groups + g.tpb - 1) / g.tpb, true, f, grp);
+    const int p0 = grp * g.tpb;
+    const int ntr = rf_block_transforms(g.tpb, groups, grp);
     const bool ydst = dst_along_y(f);
     const C *tw = static_cast<const C *>(g.tw);
     const TG *work = static_cast<const TG *>(g.work) + (size_t)f * N * N;
+    const RfDiv dper(per);
     RF_CHECK(ntr >= 1 && 2 * ntr * L <= CAP);
     for (int it = threadIdx.x; it < ntr * per; it += RF_T) {
-        const int tr = it / per, k = it - tr * per;
+        const int tr = dper.div(it), k = it - tr * per;
         const int ia = 2 * (p0 + tr), ib = ia + 1;
         RF_CHECK(ia < N && k <= M);
         TG a = work[(size_t)ia * N + k];
@@ -380,7 +429,7 @@ __global__ __launch_bounds__(RF_T) void k_rf_rows_inv(RfftGeom g, SolveOut<TFULL
     __syncthreads();
     const C *z = rf_fft<TG>(lds, lds + CAP / 2, tw, g.stages, L, ntr);
     for (int it = threadIdx.x; it < ntr * per; it += RF_T) {
-        const int tr = it / per, j = it - tr * per;
+        const int tr = dper.div(it), j = it - tr * per;
         const int ia = 2 * (p0 + tr), ib = ia + 1;
         const C v = z[tr * L + j];
         const bool zero = ydst && (j == 0 || j == M);
@@ -425,7 +474,7 @@ template <typename TG, typename TPRED, typename TFULL, typename THALF, int CAP>
 void rf_launch(const RfftGeom &g, const RhsEval<TG, TPRED> &rhs, const SolveOut<TFULL, THALF> &out,
                bool write_full, hipStream_t s)
 {
-    const dim3 grid(((g.N + 1) / 2 + g.tpb - 1) / g.tpb, 7);
+    const unsigned grid = 7 * (unsigned)(((g.N + 1) / 2 + g.tpb - 1) / g.tpb);
     k_rf_rows_fwd<TG, TPRED, CAP><<<grid, RF_T, 0, s>>>(g, rhs);
     k_rf_cols<TG, CAP><<<grid, RF_T, 0, s>>>(g);
     k_rf_rows_inv<TG, TFULL, THALF, CAP><<<grid, RF_T, 0, s>>>(g, out, write_full);
@@ -442,7 +491,7 @@ hipError_t launch_compute_fields_rfft(const RfftGeom &g, const SolveIn<TG, TPRED
         return hipErrorInvalidValue;
     const RhsEval<TG, TPRED> rhs{in, N, TG(1.0 / (2.0 * g.h)), TG(1.0 / g.xi_step), TG(g.c)};
     hipError_t e;
-    if (timer && (e = timer->begin(KC_SOLVE_GEMM, 3, s)) != hipSuccess) return e;
+    if (timer && (e = timer->begin(KC_SOLVE, 3, s)) != hipSuccess) return e;
     if (g.cap == RfCaps<TG>::c0)
         rf_launch<TG, TPRED, TFULL, THALF, RfCaps<TG>::c0>(g, rhs, out, write_full, s);
     else if (g.cap == RfCaps<TG>::c1)

@@ -306,7 +306,7 @@ hipError_t launch_compute_fields_matrix(const MatrixGeom &g, const SolveIn<TG, T
     }
     k_rhs<TG, TPRED><<<(unsigned)((nn + 255) / 256), 256, 0, s>>>(rhs, w1);
     hipError_t e;
-    if (timer && (e = timer->begin(KC_SOLVE_GEMM, 4, s)) != hipSuccess) return e;
+    if (timer && (e = timer->begin(KC_SOLVE, 4, s)) != hipSuccess) return e;
     gemm_launch<TG, false>(fy, N, sg, s);
     gemm_launch<TG, true>(sx, N, sg, s);
     gemm_launch<TG, false>(ix, N, sg, s);

@@ -31,10 +31,12 @@ class Plane2d3vPlasmaSolver:
         self.compute_rhoj = get_rhoj_computer(config)
         self.compute_fields = get_field_computer(config)
         self.move_particles_wo_fields, self.move_particles = get_plasma_particles_mover(config)
+        # The electrons' noise filter after the last push (solver.py:16-17,
+        # 69-73) runs inside the native step.
         self.declustering_enabled = config.getbool('declustering-enabled')
         if self.declustering_enabled:
-            raise NotImplementedError('declustering (noise_filter) is not available on the HIP '
-                                      'path yet')
+            from .noise_filter import get_noise_filter
+            self.noise_filter = get_noise_filter(config)
 
     def step_dxi(self, particles_prev: dict, fields_prev: Arrays, currents_prev: Arrays,
                  const_arrays: Arrays, rho_beam_full, rho_beam_prev):
@@ -54,6 +56,16 @@ class Plane2d3vPlasmaSolver:
                    if k not in _bind.PARTICLE_KEYS})
             particles_full[sort] = full
             structs[i] = _bind.sort_struct(prev, full, column)
+            if self.declustering_enabled and i == 0:
+                z = lambda: torch.zeros_like(prev.x_offt, dtype=torch.float64)
+                dxp = getattr(prev, 'dx_chaotic', None)
+                dyp = getattr(prev, 'dy_chaotic', None)
+                dxp = z() if dxp is None else dxp.to(torch.float64).contiguous()
+                dyp = z() if dyp is None else dyp.to(torch.float64).contiguous()
+                full.dx_chaotic, full.dy_chaotic = z(), z()
+                s = structs[i]
+                s.dx_chaotic_prev, s.dy_chaotic_prev = dxp.data_ptr(), dyp.data_ptr()
+                s.dx_chaotic, s.dy_chaotic = full.dx_chaotic.data_ptr(), full.dy_chaotic.data_ptr()
         fields_full = new_fields(N, dt.full)
         fields_half = new_fields(N, dt.half)
         cur = Arrays.wrap(**{k: torch.empty(2, N, N, dtype=dt.current, device=dev)

@@ -7,6 +7,7 @@ config (plasma3d/solver.py:10-17).
 """
 from __future__ import annotations
 
+import math
 import threading
 
 import numpy as np
@@ -70,8 +71,12 @@ def geometry(config: Config) -> dict:
                 lost_radius=max(0.9 * max_radius, max_radius - 1))
 
 
-def context(config: Config, max_particles: int | None = None) -> _native.Context:
-    """Shared context for an adjusted 3D config."""
+def context(config: Config, max_particles: int | None = None,
+            private: bool = False) -> _native.Context:
+    """Shared context for an adjusted 3D config; with ``private`` a new one
+    that nothing else uses (work of several contexts may then run
+    concurrently on different streams: a context's workspaces serve one
+    stream at a time)."""
     import torch
     config.check_hip_backend()
     g = geometry(config)
@@ -82,14 +87,28 @@ def context(config: Config, max_particles: int | None = None) -> _native.Context
     dev = device().index
     solver = config.field_solver()
     shape = (int(config.get('rfft-cap', 0) or 0), int(config.get('rfft-tpb', 0) or 0))
-    key = (dev, prec, radix, solver, shape) + tuple(sorted(g.items()))
+    if solver == 'auto':
+        from . import autotune
+        choice = autotune.solve_choice(dev, g['N'], g['h'], g['xi_step'],
+                                       g['subtraction_coeff'], prec)
+        solver, radix = choice['field_solver'], choice['fft_radix']
+        shape = (choice['rfft_cap'], choice['rfft_tpb'])
+    decl = None
+    if config.getbool('declustering-enabled'):
+        from .plasma3d.noise_filter import declustering_params
+        side = math.isqrt(lattice_size(config))
+        decl = declustering_params(config, side)
+    dkey = None if decl is None else (decl['side'], config.getint('declustering-averaging'),
+                                      decl['force'], decl['damping'], decl['limit'])
+    key = (dev, prec, radix, solver, shape, dkey) + tuple(sorted(g.items()))
     with _LOCK:
-        ctx = _CACHE.get(key)
+        ctx = None if private else _CACHE.get(key)
         if ctx is None or ctx.max_particles < cap:
             with torch.cuda.device(dev):
                 ctx = _native.Context(eigenvalues=eigenvalues(g['N'], g['h']), precision=prec,
                                       max_particles=cap, fft_radix=radix,
                                       field_solver=solver, rfft_cap=shape[0],
-                                      rfft_tpb=shape[1], **g)
-            _CACHE[key] = ctx
+                                      rfft_tpb=shape[1], declustering=decl, **g)
+            if not private:
+                _CACHE[key] = ctx
     return ctx

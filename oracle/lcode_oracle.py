@@ -20,13 +20,21 @@ It restates, in numpy plus the C kernels of ``oracle/kernels.c``:
 * the rigid beam                -- beam3d/beam_calculator.py:148-181
 * the time loop                 -- simulation.py:244-264, mpi/beam_io.py:81-119
 * the energy flux diagnostic    -- diagnostics/FXi/Strategy3D.py:6-25
+* the declustering noise filter -- plasma3d/noise_filter.py:53-238 (CPU
+                                   branch: numba inhomogeneity, scipy.ndimage
+                                   fourier_gaussian) and its place in
+                                   plasma3d/solver.py:69-73
 
 Parity pins (tests/test_oracle_*.py): the reference's own end-state fixtures
 ``3D_test1.npz`` and ``3D_beam_evol.npz`` (tests/integration/test_3D_part.py,
 rtol 5e-8), closed-form eigenmodes of the three solves, DST-I/DCT-I
 self-inverse identities, spline-weight invariants and single-particle cases;
 the energy flux by closed forms (uniform fields, the cancelling Ex = By,
-Ey = -Bx combination, a single particle).
+Ey = -Bx combination, a single particle); the noise filter's inhomogeneity by
+brute force, its smoothing by scipy's fourier_gaussian (a constant stays
+constant), and whole-filter closed forms (an undisturbed or uniformly shifted
+lattice gets no kick; a single displaced particle gets the kick the
+formulas give by hand).
 """
 from __future__ import annotations
 
@@ -94,6 +102,7 @@ class Params:
     time_step: float = 1.0
     substepping_energy: float = 2.0
     nthreads: int = 0           # 0: all OpenMP threads
+    declustering: 'Declustering | None' = None   # noise filter (None: off)
 
     @property
     def boundary(self) -> float:   # plasma3d/move.py:321
@@ -112,6 +121,7 @@ class Params:
     def from_config(cls, config, nthreads: int = 0) -> 'Params':
         """From an adjusted lcodepy_amd Config (same keys as the reference)."""
         fin = config.getfloat('plasma-fineness')
+        decl = Declustering.from_config(config) if config.getbool('declustering-enabled') else None
         return cls(N=config.getint('window-width-steps'),
                    h=config.getfloat('transverse-step'),
                    xi_step=config.getfloat('xi-step'),
@@ -124,7 +134,7 @@ class Params:
                    window_length=config.getfloat('window-length'),
                    time_step=config.getfloat('time-step'),
                    substepping_energy=config.getfloat('beam-substepping-energy'),
-                   nthreads=nthreads)
+                   nthreads=nthreads, declustering=decl)
 
 
 # --------------------------------------------------------------------------
@@ -399,8 +409,10 @@ def compute_fields(p: Params, fields, fields_prev, const: Const, ro_beam_full, r
 
 def step_dxi(p: Params, particles_prev, fields_prev, currents_prev, const: Const,
              ro_beam_full, ro_beam_prev, trace: dict | None = None):
-    """plasma3d/solver.py:25-78 (declustering off).  If ``trace`` is a dict
-    the intermediate stages are stored in it for stage-by-stage checks."""
+    """plasma3d/solver.py:25-78, with the electrons' noise filter after the
+    last push when p.declustering is set (solver.py:69-73).  If ``trace`` is
+    a dict the intermediate stages are stored in it for stage-by-stage
+    checks."""
     full = move_wo_fields(p, particles_prev)
     if trace is not None:
         trace['free'] = {s: {k: v.copy() for k, v in a.items()} for s, a in full.items()}
@@ -424,8 +436,89 @@ def step_dxi(p: Params, particles_prev, fields_prev, currents_prev, const: Const
         trace['half2'] = dict(fields_half)
         trace['full2'] = dict(fields_full)
     full = move_smart(p, fields_half, particles_prev, full)
+    if p.declustering is not None:
+        full['electrons'] = noise_filter(p, full['electrons'], particles_prev['electrons'])
     currents_full = compute_rhoj(p, full, const)
     return full, fields_full, currents_full
+
+
+# --------------------------------------------------------------------------
+# Declustering noise filter (plasma3d/noise_filter.py)
+# --------------------------------------------------------------------------
+@dataclass
+class Declustering:
+    """The filter's parameters as get_noise_filter reads them
+    (noise_filter.py:82-90).  The damping comes from the key
+    'damping-declustering' (noise_filter.py:89), not the default's
+    'declustering-damping' (default_config_values.py:63), so it is 0 unless
+    that spelling is set."""
+    averaging: float = 3       # 'declustering-averaging', read with getint
+    force: float = 0.3         # 'declustering-force'
+    damping: float = 0.0       # 'damping-declustering'
+    limit: float = 1e-3        # 'declustering-limit'
+
+    @classmethod
+    def from_config(cls, config) -> 'Declustering':
+        return cls(averaging=config.getint('declustering-averaging'),
+                   force=config.getfloat('declustering-force'),
+                   damping=config.getfloat('damping-declustering'),
+                   limit=config.getfloat('declustering-limit'))
+
+
+def inhomogeneity(d: np.ndarray, axis: int) -> np.ndarray:
+    """calculate_inhomogeneity_numba (noise_filter.py:53-76) of a square
+    lattice array: along the second index (axis 0) or the first (axis 1),
+    d[j] - (d[j-1] + d[j+1]) / 2 inside and d[0] - 2 d[1] + d[2] (mirrored
+    at the far end) on the edges, where all three values are non-zero;
+    0 elsewhere."""
+    if axis:
+        d = d.T
+    out = np.zeros_like(d)
+    mid = (d[:, 1:-1] != 0) & (d[:, :-2] != 0) & (d[:, 2:] != 0)
+    out[:, 1:-1] = np.where(mid, d[:, 1:-1] - (d[:, :-2] + d[:, 2:]) / 2, 0.0)
+    m0 = (d[:, 0] != 0) * (d[:, 1] != 0) * (d[:, 2] != 0)
+    out[m0, 0] = d[m0, 0] - 2 * d[m0, 1] + d[m0, 2]
+    m1 = (d[:, -1] != 0) * (d[:, -2] != 0) * (d[:, -3] != 0)
+    out[m1, -1] = d[m1, -1] - 2 * d[m1, -2] + d[m1, -3]
+    return out.T if axis else out
+
+
+def gaussian_smooth(a: np.ndarray, sigma: float) -> np.ndarray:
+    """ifft2(fourier_gaussian(fft2(a), sigma)).real (noise_filter.py:147-148)."""
+    from scipy.ndimage import fourier_gaussian
+    return np.fft.ifft2(fourier_gaussian(np.fft.fft2(a), sigma=sigma)).real
+
+
+def noise_filter(p: Params, el: dict, el_prev: dict) -> dict:
+    """noise_filter(particles, particles_prev) (noise_filter.py:102-238) on
+    one sort's (n, n) lattice arrays: kicks px and py in place and returns
+    the new dx_chaotic, dy_chaotic (and the other arrays unchanged)."""
+    f = p.declustering
+    xi = p.xi_step
+    shape = el['x_offt'].shape
+    zero = np.zeros(shape)
+    dxp = el_prev.get('dx_chaotic', zero)
+    dyp = el_prev.get('dy_chaotic', zero)
+    x_offt, y_offt, px, py = el['x_offt'], el['y_offt'], el['px'], el['py']
+    dx_inhom = (inhomogeneity(x_offt, 0) + inhomogeneity(x_offt, 1)) / 2
+    dy_inhom = (inhomogeneity(y_offt, 1) + inhomogeneity(y_offt, 0)) / 2
+    dx_chaotic = dx_inhom - gaussian_smooth(dx_inhom, f.averaging)
+    dy_chaotic = dy_inhom - gaussian_smooth(dy_inhom, f.averaging)
+    d_limit = dx_chaotic * dx_chaotic + dy_chaotic * dy_chaotic
+    d_limit = np.where(d_limit < f.limit ** 2, 1 - d_limit / f.limit ** 2, 0)
+    force_x = - f.force * dx_chaotic * d_limit
+    force_y = - f.force * dy_chaotic * d_limit
+    for pp, force, chaotic, prev in ((px, force_x, dx_chaotic, dxp), (py, force_y, dy_chaotic, dyp)):
+        pp[1:-1, :] += xi * force[1:-1, :] / 2
+        pp[2:, :] -= xi * force[1:-1, :] / 4
+        pp[:-2, :] -= xi * force[1:-1, :] / 4
+        pp[:, 1:-1] += xi * force[:, 1:-1] / 2
+        pp[:, 2:] -= xi * force[:, 1:-1] / 4
+        pp[:, :-2] -= xi * force[:, 1:-1] / 4
+        pp -= xi * f.damping * d_limit * (chaotic - prev)
+    out = dict(el)
+    out['dx_chaotic'], out['dy_chaotic'] = dx_chaotic, dy_chaotic
+    return out
 
 
 # --------------------------------------------------------------------------

@@ -170,5 +170,19 @@ int main(int argc, char **argv) {
             if (!(e64 < 1e-12) || !(e32 < 1e-4)) { printf("FAIL solver mismatch\n"); return 1; }
         }
     }
+    {   // the solve autotuner over every candidate at the first size
+        const int N = argc > 1 ? atoi(argv[1]) : 33;
+        lc3d_params p{}; p.grid_steps = N; p.grid_step_size = 0.1; p.xi_step_size = 0.05;
+        p.boundary = 0.1 * (N / 2.0 - 10); p.subtraction_coeff = 1.0; p.lost_radius = 1.0;
+        for (int prec : {LC3D_FP64, LC3D_FP32_BULK}) {
+            p.precision = prec;
+            lc3d_solve_choice best{}, table[128]; int32_t n = 0;
+            CHECK(lc3d_autotune_solve(&p, 1, &best, table, 128, &n));
+            int refused = 0;
+            for (int i = 0; i < n; ++i) refused += table[i].seconds < 0;
+            printf("autotune N=%d prec=%d: %d candidates, %d refused, best solver %d\n", N, prec, n, refused, best.field_solver);
+            if (n < 2 || refused) { printf("FAIL autotune\n"); return 1; }
+        }
+    }
     printf("ALL OK\n");
 }

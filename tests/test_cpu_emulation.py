@@ -64,7 +64,9 @@ def test_gpu_parity_suite_on_cpu_build(emu_build):
            'tests/test_gpu_kernels.py::test_push_matches_oracle[65-fp64]',
            'tests/test_gpu_kernels.py::test_push_matches_oracle[65-fp32-bulk]',
            'tests/test_gpu_kernels.py::test_push_reflects_and_absorbs_at_the_wall',
-           'tests/test_gpu_kernels.py::test_plasma_step_matches_oracle[33-mobile]']
+           'tests/test_gpu_kernels.py::test_plasma_step_matches_oracle[33-mobile]',
+           'tests/test_gpu_declustering.py::test_noise_filter_matches_oracle[fp64-0.6]',
+           'tests/test_gpu_declustering.py::test_noise_filter_matches_oracle[fp32-bulk-0.6]']
     env = _env()
     env['LD_PRELOAD'] = _libasan()
     r = subprocess.run(['python3', os.path.join(EMU, 'run_parity.py')] + ids,
