@@ -255,6 +255,16 @@ int lc3d_move_beam(lc3d_ctx *ctx, int64_t n, int64_t beam_layer_idx, const doubl
                    double *px, double *py, double *pz, uint8_t *lost, uint8_t *moved,
                    uint8_t *fell, void *stream);
 
+/* The split of a moved layer into (lost, moved, fell), beam_calculator.py:
+ * 124-139 (the three boolean-mask gathers of move_beam_layer): f holds nf
+ * float64 rows and iv ni int64 rows of n particles (row r at f + r n),
+ * flags three uint8 rows (lost, moved, fell; what lc3d_move_beam wrote).
+ * Group g's particles go, in layer order, to the front of the rows of
+ * out_f + g nf n and out_i + g ni n; their number to counts[g] (device
+ * int64[3]). */
+int lc3d_beam_split(lc3d_ctx *ctx, int64_t n, int32_t nf, int32_t ni, const double *f,
+                    const int64_t *iv, const uint8_t *flags, double *out_f, int64_t *out_i,
+                    int64_t *counts, void *stream);
 /* Total energy flux of a plasma layer, diagnostics/FXi/Strategy3D.py:6-25
  * (calculate_energy_fluxes): the sum over the grid of (Ez^2 + Bz^2 +
  * (Ex - By)^2 + (Ey + Bx)^2) h^2 / 2 plus the sum over the n electrons of

@@ -104,7 +104,7 @@ SYMBOLS = ('lc3d_create', 'lc3d_destroy', 'lc3d_last_error', 'lc3d_precision',
            'lc3d_compute_fields', 'lc3d_plasma_step', 'lc3d_deposit_beam',
            'lc3d_beam_substep', 'lc3d_move_beam', 'lc3d_energy_flux', 'lc3d_probe_sample',
            'lc3d_abs_max', 'lc3d_energy_flux_noalloc', 'lc3d_autotune_solve',
-           'lc3d_noise_filter')
+           'lc3d_noise_filter', 'lc3d_beam_split')
 
 _LIB = None
 _LOCK = threading.Lock()
@@ -143,6 +143,7 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
     L.lc3d_autotune_solve.argtypes = [P(Params), i32, P(SolveChoice), P(SolveChoice), i32,
                                       P(i32)]
     L.lc3d_noise_filter.argtypes = [_vp, P(Sort), _vp]
+    L.lc3d_beam_split.argtypes = [_vp, i64, i32, i32, _vp, _vp, _vp, _vp, _vp, _vp, _vp]
     L.lc3d_probe_sample.argtypes = [_vp, P(ProbeSources), i32, _vp, _vp, _vp, _vp]
     L.lc3d_abs_max.argtypes = [_vp, _vp, i64, i64, i64, i64, i64, i64, _vp, _vp]
     L.lc3d_move_beam.argtypes = [_vp, i64, i64, _vp, _vp, P(Fields6), P(Fields6), _vp, _vp,

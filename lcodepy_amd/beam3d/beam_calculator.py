@@ -68,20 +68,20 @@ class BeamCalculator:
         if n == 0:
             return PendingMove(None, None, None, None)
         beam_layer.contiguous()
-        flags = torch.zeros(3, n, dtype=torch.bool, device=dev)   # lost, moved, fell
+        # lost, moved, fell: every entry is written by the move
+        flags = torch.empty(3, n, dtype=torch.uint8, device=dev)
         self.start_moving_layer(beam_layer, fell_size)
         self.move_particles(pl_layer_idx - 1, beam_layer, fields_after_layer,
                             fields_before_layer, flags[0], flags[1], flags[2])
-        # Selected particle k goes to column cumsum(flag)[k] - 1 of its
-        # group, the others to the spare column n (dropped by the slice).
-        dest = torch.where(flags, torch.cumsum(flags, dim=1) - 1, n)
-        out_f = beam_layer.f.new_empty(3, beam_layer.f.shape[0], n + 1)
-        out_i = beam_layer.i.new_empty(3, beam_layer.i.shape[0], n + 1)
-        for g in range(3):
-            out_f[g].index_copy_(1, dest[g], beam_layer.f)
-            out_i[g].index_copy_(1, dest[g], beam_layer.i)
+        nf, ni = beam_layer.f.shape[0], beam_layer.i.shape[0]
+        out_f = beam_layer.f.new_empty(3, nf, n)
+        out_i = beam_layer.i.new_empty(3, ni, n)
+        counts_dev = torch.empty(3, dtype=torch.int64, device=dev)
+        self.ctx.check(self.ctx.lib.lc3d_beam_split(
+            self.ctx.handle, n, nf, ni, ptr(beam_layer.f), ptr(beam_layer.i), ptr(flags),
+            ptr(out_f), ptr(out_i), ptr(counts_dev), stream_ptr()), 'lc3d_beam_split')
         counts = torch.empty(3, dtype=torch.int64, pin_memory=True)
-        counts.copy_(flags.sum(dim=1), non_blocking=True)
+        counts.copy_(counts_dev, non_blocking=True)
         done = torch.cuda.Event()
         done.record()
         return PendingMove(out_f, out_i, counts, done)

@@ -835,6 +835,19 @@ int lc3d_move_beam(lc3d_ctx *c, int64_t n, int64_t beam_layer_idx, const double 
     return LC3D_OK;
 }
 
+int lc3d_beam_split(lc3d_ctx *c, int64_t n, int32_t nf, int32_t ni, const double *f,
+                    const int64_t *iv, const uint8_t *flags, double *out_f, int64_t *out_i,
+                    int64_t *counts, void *stream)
+{
+    if (!c || n < 0 || nf < 0 || ni < 0 || !counts ||
+        (n > 0 && ((nf > 0 && (!f || !out_f)) || (ni > 0 && (!iv || !out_i)) || !flags)))
+        return fail(c, LC3D_EINVAL, "beam_split: bad arguments");
+    hipError_t e = launch_beam_split(n, nf, ni, f, iv, flags, out_f, out_i, counts,
+                                     static_cast<hipStream_t>(stream));
+    if (e != hipSuccess) return hip_fail(c, e, "beam_split");
+    return LC3D_OK;
+}
+
 int lc3d_energy_flux(lc3d_ctx *c, const lc3d_fields6 *f, int64_t n, const void *m,
                      const void *px, const void *py, const double *pz, double *psi, void *stream)
 {

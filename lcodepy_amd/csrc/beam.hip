@@ -1,32 +1,43 @@
 // Beam layer kernels (beam3d), gfx950.
 //
 // A beam layer is small (thousands of macro-particles per xi step against
-// ~10^6 plasma particles), so the beam deposit is a direct per-particle
-// scatter with float64 atomics; the push is one thread per particle running
-// the reference's CPU (explicit-midpoint) scheme with its substep loop.
+// ~10^6 plasma particles): one particle per thread would leave most of the
+// GPU idle and each thread a long chain of dependent field loads, so a group
+// of BEAM_LANES lanes serves one particle, each lane taking every
+// BEAM_LANES-th of the 25 stencil taps.  The deposit adds each tap with a
+// float64 atomic; the push (the reference's CPU explicit-midpoint scheme with
+// its substep loop) sums the lanes' tap contributions by a butterfly over
+// the group, which leaves the same bits in every lane, and every lane then
+// runs the same momentum update.
 #include "common.h"
 #include "kernels.h"
 
 namespace lc3d {
 
 constexpr int MAX_BEAM_SUBSTEPS = 1 << 16;
+constexpr int BEAM_LANES = 8;   // lanes per beam particle (a power of two dividing 64)
+constexpr int BEAM_THREADS = 256;
 
 // beam3d/weights.py:40-64 with x_h and xi_loc from :152-160.  ro0 is the
 // running layer (weight xi_loc), ro1 the next one (weight 1 - xi_loc).
-__global__ __launch_bounds__(256) void k_beam_deposit(BeamGeom g, int64_t n, double xi_layer,
-                                                      const double *__restrict__ x,
-                                                      const double *__restrict__ y,
-                                                      const double *__restrict__ xi,
-                                                      const double *__restrict__ q_norm,
-                                                      double *__restrict__ ro0,
-                                                      double *__restrict__ ro1, unsigned *err)
+__global__ __launch_bounds__(BEAM_THREADS) void k_beam_deposit(BeamGeom g, int64_t n,
+                                                               double xi_layer,
+                                                               const double *__restrict__ x,
+                                                               const double *__restrict__ y,
+                                                               const double *__restrict__ xi,
+                                                               const double *__restrict__ q_norm,
+                                                               double *__restrict__ ro0,
+                                                               double *__restrict__ ro1,
+                                                               unsigned *err)
 {
-    const int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    const int64_t t = blockIdx.x * (int64_t)BEAM_THREADS + threadIdx.x;
+    const int64_t k = t / BEAM_LANES;
+    const int lane = (int)(t % BEAM_LANES);
     if (k >= n) return;
     const int N = g.N;
     const double xh = x[k] / g.h + 0.5, yh = y[k] / g.h + 0.5;
     if (!isfinite(xh) || !isfinite(yh)) {
-        atomicOr(err, ERR_BEAM_OFF_GRID);
+        if (lane == 0) atomicOr(err, ERR_BEAM_OFF_GRID);
         return;
     }
     const double fx = floor(xh), fy = floor(yh);
@@ -38,20 +49,16 @@ __global__ __launch_bounds__(256) void k_beam_deposit(BeamGeom g, int64_t n, dou
     weights4<double>(yh - fy - 0.5, wy);
     const double q = q_norm[k];
     bool off = false;
-#pragma unroll
-    for (int kx = 0; kx < 5; ++kx) {
-        const int gx = ix + kx - 2;
-#pragma unroll
-        for (int ky = 0; ky < 5; ++ky) {
-            const int gy = iy + ky - 2;
-            if (gx < 0 || gx >= N || gy < 0 || gy >= N) {
-                off = true;
-                continue;
-            }
-            const double w = wx[kx] * wy[ky];
-            atomicAdd(&ro0[gx * N + gy], q * (w * xi_loc));
-            atomicAdd(&ro1[gx * N + gy], q * (w * (1 - xi_loc)));
+    for (int tap = lane; tap < 25; tap += BEAM_LANES) {
+        const int kx = tap / 5, ky = tap - 5 * kx;
+        const int gx = ix + kx - 2, gy = iy + ky - 2;
+        if (gx < 0 || gx >= N || gy < 0 || gy >= N) {
+            off = true;
+            continue;
         }
+        const double w = wx[kx] * wy[ky];
+        atomicAdd(&ro0[gx * N + gy], q * (w * xi_loc));
+        atomicAdd(&ro1[gx * N + gy], q * (w * (1 - xi_loc)));
     }
     if (off) atomicOr(err, ERR_BEAM_OFF_GRID);
 }
@@ -74,38 +81,44 @@ __global__ __launch_bounds__(256) void k_beam_substep(int64_t n, const double *_
     steps[k] = (int64_t)(1. / d);
 }
 
+// This lane's share of the two-layer field gather of beam3d/move.py:70-91
+// (taps lane, lane + BEAM_LANES, ...), summed over the particle's lanes.
 template <typename TF>
 __device__ __forceinline__ void gather2(const Fields6<TF> &a, const Fields6<TF> &b, int N, int ix,
                                         int iy, const double wx[5], const double wy[5], double w0s,
-                                        double w1s, double F[6], bool &off)
+                                        double w1s, int lane, double F[6], bool &off)
 {
-#pragma unroll
-    for (int kx = 0; kx < 5; ++kx) {
-        const int gx = ix + kx - 2;
-#pragma unroll
-        for (int ky = 0; ky < 5; ++ky) {
-            const int gy = iy + ky - 2;
-            if (gx < 0 || gx >= N || gy < 0 || gy >= N) {
-                off = true;
-                continue;
-            }
-            const int o = gx * N + gy;
-            const double w = wx[kx] * wy[ky];
-            const double w0 = w * w0s, w1 = w * w1s;
-            F[0] += (double)a.Ex[o] * w0 + (double)b.Ex[o] * w1;
-            F[1] += (double)a.Ey[o] * w0 + (double)b.Ey[o] * w1;
-            F[2] += (double)a.Ez[o] * w0 + (double)b.Ez[o] * w1;
-            F[3] += (double)a.Bx[o] * w0 + (double)b.Bx[o] * w1;
-            F[4] += (double)a.By[o] * w0 + (double)b.By[o] * w1;
-            F[5] += (double)a.Bz[o] * w0 + (double)b.Bz[o] * w1;
+    for (int tap = lane; tap < 25; tap += BEAM_LANES) {
+        const int kx = tap / 5, ky = tap - 5 * kx;
+        const int gx = ix + kx - 2, gy = iy + ky - 2;
+        if (gx < 0 || gx >= N || gy < 0 || gy >= N) {
+            off = true;
+            continue;
         }
+        const int o = gx * N + gy;
+        const double w = wx[kx] * wy[ky];
+        const double w0 = w * w0s, w1 = w * w1s;
+        F[0] += (double)a.Ex[o] * w0 + (double)b.Ex[o] * w1;
+        F[1] += (double)a.Ey[o] * w0 + (double)b.Ey[o] * w1;
+        F[2] += (double)a.Ez[o] * w0 + (double)b.Ez[o] * w1;
+        F[3] += (double)a.Bx[o] * w0 + (double)b.Bx[o] * w1;
+        F[4] += (double)a.By[o] * w0 + (double)b.By[o] * w1;
+        F[5] += (double)a.Bz[o] * w0 + (double)b.Bz[o] * w1;
+    }
+#pragma unroll
+    for (int off_l = 1; off_l < BEAM_LANES; off_l <<= 1) {
+#pragma unroll
+        for (int c = 0; c < 6; ++c) F[c] += __shfl_xor(F[c], off_l, BEAM_LANES);
     }
 }
 
 // beam3d/move.py:31-127 (the CPU scheme; its trailing "fell" test compares
 // x, not xi, as move.py:123 does).
+// A group of BEAM_LANES lanes per particle; all lanes of a group run the
+// same substep loop (the same particle state), so the group's shuffles see
+// every lane of it; lane 0 writes the particle back.
 template <typename TF>
-__global__ __launch_bounds__(256) void k_beam_move(
+__global__ __launch_bounds__(BEAM_THREADS) void k_beam_move(
     BeamGeom g, int64_t n, double xi_k, double xi_k1, const double *__restrict__ q_m,
     const double *__restrict__ dt, Fields6<TF> fk, Fields6<TF> fk1, int64_t *__restrict__ steps,
     int64_t *__restrict__ id, double *__restrict__ x, double *__restrict__ y,
@@ -113,8 +126,10 @@ __global__ __launch_bounds__(256) void k_beam_move(
     double *__restrict__ pz, uint8_t *__restrict__ lost, uint8_t *__restrict__ moved,
     uint8_t *__restrict__ fell, unsigned *err)
 {
-    const int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    if (k >= n) return;
+    const int64_t t = blockIdx.x * (int64_t)BEAM_THREADS + threadIdx.x;
+    const int64_t k = t / BEAM_LANES;
+    const int lane = (int)(t % BEAM_LANES);
+    if (k >= n) return;   // whole groups: n particles fill BEAM_LANES n lanes
     const int N = g.N;
     const double r2 = g.lost_radius * g.lost_radius;
     const double qm = q_m[k], iqm = 1 / qm, d = dt[k];
@@ -142,7 +157,6 @@ __global__ __launch_bounds__(256) void k_beam_move(
             X = xh_;
             Y = yh_;
             XI = xih;
-            id[k] = -id[k];
             is_lost = true;
             rs = 0;
             break;
@@ -156,7 +170,7 @@ __global__ __launch_bounds__(256) void k_beam_move(
         weights4<double>(yhh - fy - .5, wy);
         const double xi_loc = (xih - xi_k) / g.xi_step;
         double F[6] = {0, 0, 0, 0, 0, 0};
-        gather2<TF>(fk, fk1, N, ix, iy, wx, wy, xi_loc, 1 - xi_loc, F, off);
+        gather2<TF>(fk, fk1, N, ix, iy, wx, wy, xi_loc, 1 - xi_loc, lane, F, off);
         const double vx = PX / gm, vy = PY / gm, vz = PZ / gm;
         const double a = sgn * d / 2;
         const double pxh = PX + a * (F[0] + vy * F[5] - vz * F[4]);
@@ -170,14 +184,16 @@ __global__ __launch_bounds__(256) void k_beam_move(
         PY = 2 * pyh - PY;
         PZ = 2 * pzh - PZ;
         if (X * X + Y * Y >= r2) {
-            id[k] = -id[k];
             is_lost = true;
             rs = 0;
             break;
         }
         rs -= 1;
     }
+    if (off) atomicOr(err, ERR_BEAM_OFF_GRID);
+    if (lane != 0) return;
     if (rs == 0 && xi_k1 > X) is_fell = true;
+    if (is_lost) id[k] = -id[k];
     x[k] = X;
     y[k] = Y;
     xi[k] = XI;
@@ -188,7 +204,63 @@ __global__ __launch_bounds__(256) void k_beam_move(
     lost[k] = is_lost;
     fell[k] = is_fell;
     moved[k] = !is_fell && !is_lost;
-    if (off) atomicOr(err, ERR_BEAM_OFF_GRID);
+}
+
+// beam_calculator.py:124-139 (the split of a moved layer): one block per
+// group g (lost, moved, fell) copies the particles whose flag[g] is set, in
+// layer order, to the front of out_f[g] / out_i[g] and writes their number
+// to counts[g].  Chunks of the block's size: a wave's selected lanes are
+// ranked by a ballot, the waves' totals by one thread.
+constexpr int SPLIT_THREADS = 1024;
+__global__ __launch_bounds__(SPLIT_THREADS) void k_beam_split(int64_t n, int nf, int ni,
+                                                              const double *__restrict__ f,
+                                                              const int64_t *__restrict__ iv,
+                                                              const uint8_t *__restrict__ flags,
+                                                              double *__restrict__ out_f,
+                                                              int64_t *__restrict__ out_i,
+                                                              int64_t *__restrict__ counts)
+{
+    __shared__ int wave_off[SPLIT_THREADS / 64 + 1];
+    const int g = blockIdx.x;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const uint8_t *fl = flags + (int64_t)g * n;
+    double *of = out_f + (int64_t)g * nf * n;
+    int64_t *oi = out_i + (int64_t)g * ni * n;
+    int64_t base = 0;
+    for (int64_t c0 = 0; c0 < n; c0 += SPLIT_THREADS) {
+        const int64_t k = c0 + threadIdx.x;
+        const bool sel = k < n && fl[k] != 0;
+        const unsigned long long m = __ballot(sel);
+        const int before = __builtin_popcountll(m & ((1ull << lane) - 1ull));
+        if (lane == 0) wave_off[wave] = __builtin_popcountll(m);
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            int acc = 0;
+            for (int w = 0; w < SPLIT_THREADS / 64; ++w) {
+                const int t = wave_off[w];
+                wave_off[w] = acc;
+                acc += t;
+            }
+            wave_off[SPLIT_THREADS / 64] = acc;
+        }
+        __syncthreads();
+        if (sel) {
+            const int64_t d = base + wave_off[wave] + before;
+            for (int r = 0; r < nf; ++r) of[(int64_t)r * n + d] = f[(int64_t)r * n + k];
+            for (int r = 0; r < ni; ++r) oi[(int64_t)r * n + d] = iv[(int64_t)r * n + k];
+        }
+        base += wave_off[SPLIT_THREADS / 64];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) counts[g] = base;
+}
+
+hipError_t launch_beam_split(int64_t n, int nf, int ni, const double *f, const int64_t *iv,
+                             const uint8_t *flags, double *out_f, int64_t *out_i, int64_t *counts,
+                             hipStream_t s)
+{
+    k_beam_split<<<3, SPLIT_THREADS, 0, s>>>(n, nf, ni, f, iv, flags, out_f, out_i, counts);
+    return hipGetLastError();
 }
 
 hipError_t launch_beam_deposit(const BeamGeom &g, int64_t n, int64_t plasma_layer_idx,
@@ -198,8 +270,9 @@ hipError_t launch_beam_deposit(const BeamGeom &g, int64_t n, int64_t plasma_laye
 {
     if (n <= 0) return hipSuccess;
     const double xi_layer = -g.xi_step * (double)plasma_layer_idx;
-    k_beam_deposit<<<(int)((n + 255) / 256), 256, 0, s>>>(g, n, xi_layer, x, y, xi, q_norm, ro0,
-                                                           ro1, err);
+    const int64_t blocks = (n * BEAM_LANES + BEAM_THREADS - 1) / BEAM_THREADS;
+    k_beam_deposit<<<(unsigned)blocks, BEAM_THREADS, 0, s>>>(g, n, xi_layer, x, y, xi, q_norm, ro0,
+                                                             ro1, err);
     return hipGetLastError();
 }
 
@@ -224,9 +297,10 @@ hipError_t launch_beam_move(const BeamGeom &g, int64_t n, int64_t beam_layer_idx
     if (n <= 0) return hipSuccess;
     const double xi_k = (double)beam_layer_idx * -g.xi_step;
     const double xi_k1 = (double)(beam_layer_idx + 1) * -g.xi_step;
-    k_beam_move<TF><<<(int)((n + 255) / 256), 256, 0, s>>>(g, n, xi_k, xi_k1, q_m, dt, fk, fk1,
-                                                            remaining_steps, id, x, y, xi, px, py,
-                                                            pz, lost, moved, fell, err);
+    const int64_t blocks = (n * BEAM_LANES + BEAM_THREADS - 1) / BEAM_THREADS;
+    k_beam_move<TF><<<(unsigned)blocks, BEAM_THREADS, 0, s>>>(g, n, xi_k, xi_k1, q_m, dt, fk, fk1,
+                                                              remaining_steps, id, x, y, xi, px,
+                                                              py, pz, lost, moved, fell, err);
     return hipGetLastError();
 }
 

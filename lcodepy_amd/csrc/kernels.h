@@ -296,4 +296,12 @@ hipError_t launch_beam_move(const BeamGeom &g, int64_t n, int64_t beam_layer_idx
                             uint8_t *lost, uint8_t *moved, uint8_t *fell, unsigned *err,
                             hipStream_t s);
 
+// Order-preserving split of a moved layer's nf float64 and ni int64 rows (n
+// particles each) by the three flag rows (lost, moved, fell): group g's
+// particles to the front of out_f[g] (nf x n) and out_i[g] (ni x n), their
+// number to counts[g].
+hipError_t launch_beam_split(int64_t n, int nf, int ni, const double *f, const int64_t *iv,
+                             const uint8_t *flags, double *out_f, int64_t *out_i, int64_t *counts,
+                             hipStream_t s);
+
 }  // namespace lc3d

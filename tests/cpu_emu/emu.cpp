@@ -24,9 +24,9 @@ thread_local unsigned char *emu_lds = nullptr;
 
 namespace {
 constexpr size_t STACK = 256 * 1024;
-enum { RUN, WAIT_BLOCK, WAIT_WAVE, DONE };
+enum { RUN, WAIT_BLOCK, WAIT_WAVE, WAIT_GROUP, DONE };
 
-struct Fiber { ucontext_t ctx; int state; unsigned gen; char *stack; };
+struct Fiber { ucontext_t ctx; int state; unsigned gen; char *stack; unsigned group = 0; };
 
 struct BlockRun {
     std::vector<Fiber> f;
@@ -34,6 +34,9 @@ struct BlockRun {
     unsigned nt = 0, cur = 0;
     unsigned block_arrived = 0, block_gen = 0;
     unsigned wave_arrived[16] = {}, wave_gen[16] = {};
+    // groups of `width` consecutive threads (width-limited shuffles), keyed
+    // by the group's first thread
+    unsigned group_arrived[1024] = {}, group_gen[1024] = {};
     const std::function<void()> *body = nullptr;
 };
 thread_local BlockRun R;
@@ -52,7 +55,7 @@ void ensure_stacks(unsigned nt)
                                MAP_PRIVATE | MAP_ANONYMOUS | MAP_NORESERVE, -1, 0);
         if (m == MAP_FAILED) { perror("emu: mmap"); abort(); }
         mprotect(m, page, PROT_NONE);   // guard page below the stack
-        R.f.push_back(Fiber{{}, DONE, 0, m + page});
+        R.f.push_back(Fiber{{}, DONE, 0, m + page, 0});
     }
 }
 
@@ -63,6 +66,7 @@ void run_block(const std::function<void()> &body, dim3 b, unsigned nt, size_t ld
     R.body = &body;
     R.block_arrived = R.block_gen = 0;
     for (int w = 0; w < 16; ++w) R.wave_arrived[w] = R.wave_gen[w] = 0;
+    for (int g = 0; g < 1024; ++g) R.group_arrived[g] = R.group_gen[g] = 0;
     blockIdx = b;
     emu_lds = lds ? (unsigned char *)std::aligned_alloc(16, (lds + 15) / 16 * 16) : nullptr;
     for (unsigned t = 0; t < nt; ++t) {
@@ -81,7 +85,8 @@ void run_block(const std::function<void()> &body, dim3 b, unsigned nt, size_t ld
         for (unsigned t = 0; t < nt; ++t) {
             Fiber &f = R.f[t];
             bool ready = f.state == RUN || (f.state == WAIT_BLOCK && R.block_gen != f.gen) ||
-                         (f.state == WAIT_WAVE && R.wave_gen[t / 64] != f.gen);
+                         (f.state == WAIT_WAVE && R.wave_gen[t / 64] != f.gen) ||
+                         (f.state == WAIT_GROUP && R.group_gen[f.group] != f.gen);
             if (!ready) continue;
             f.state = RUN;
             R.cur = t;
@@ -167,6 +172,22 @@ void emu_sync_wave()
     }
     me.state = WAIT_WAVE;
     me.gen = R.wave_gen[w];
+    swapcontext(&me.ctx, &R.sched);
+}
+
+void emu_sync_group(unsigned width)
+{
+    Fiber &me = R.f[R.cur];
+    const unsigned first = R.cur / width * width;
+    const unsigned n = std::min(width, R.nt - first);
+    if (++R.group_arrived[first] == n) {
+        R.group_arrived[first] = 0;
+        ++R.group_gen[first];
+        return;
+    }
+    me.state = WAIT_GROUP;
+    me.group = first;
+    me.gen = R.group_gen[first];
     swapcontext(&me.ctx, &R.sched);
 }
 

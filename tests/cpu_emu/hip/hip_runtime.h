@@ -56,6 +56,7 @@ extern thread_local float emu_shfl_f[1024];
 extern thread_local int emu_shfl_i[1024];
 void emu_sync_block();
 void emu_sync_wave();
+void emu_sync_group(unsigned width);
 inline void __syncthreads() { emu_sync_block(); }
 inline double __shfl_xor(double v, int off) {
     int t = threadIdx.x, w = t / 64;
@@ -90,6 +91,16 @@ inline unsigned long long __ballot(int pred) {
         if (emu_shfl_i[w * 64 + l]) m |= 1ull << l;
     emu_sync_wave();
     return m;
+}
+// Width-limited form: only the `width` lanes of the caller's group take part
+// (and must all reach it); off < width keeps the partner in the group.
+inline double __shfl_xor(double v, int off, int width) {
+    int t = threadIdx.x, base = t / width * width;
+    emu_shfl_d[t] = v;
+    emu_sync_group(width);
+    double r = emu_shfl_d[base + ((t - base) ^ off)];
+    emu_sync_group(width);
+    return r;
 }
 inline float __shfl_xor(float v, int off) {
     int t = threadIdx.x, w = t / 64;
