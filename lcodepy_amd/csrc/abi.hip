@@ -26,9 +26,9 @@ struct lc3d_ctx {
     FftPlan plan{};
     // binning
     int *count = nullptr, *start = nullptr, *blocks = nullptr, *key = nullptr, *idx = nullptr;
-    int *rank = nullptr;
+    int *rank = nullptr, *pid = nullptr, *pos = nullptr;
     unsigned *err = nullptr;
-    void *rec = nullptr, *rec_sorted = nullptr;
+    void *rec = nullptr;
     int *big = nullptr, *nbig = nullptr, *dslot = nullptr;
     void *dense = nullptr;
     int bigcap = 0;
@@ -160,8 +160,9 @@ PushGeom push_geom(const lc3d_ctx *c, int64_t n)
 BinState bin_state(const lc3d_ctx *c, int col)
 {
     const int64_t cap = std::max<int64_t>(1, c->cap);
-    return BinState{c->count, c->start, c->blocks, c->key, c->rank, c->idx + col * cap, c->err,
-                    c->rec, cap, c->big, c->nbig, c->bigcap, c->dslot, c->dense, c->rec_sorted};
+    return BinState{c->count, c->start, c->blocks, c->key,    c->rank,  c->pid,
+                    c->pos,   c->idx + col * cap, c->err, c->rec, cap,     c->big,
+                    c->nbig,  c->bigcap,          c->dslot, c->dense};
 }
 
 template <typename TP> SortArrays<TP> sort_arrays(const lc3d_sort *s)
@@ -493,7 +494,8 @@ int lc3d_create(const lc3d_params *p, lc3d_ctx **out)
               hipMalloc(&c->idx, 2 * cap * sizeof(int)) == hipSuccess &&
               hipMalloc(&c->err, sizeof(unsigned)) == hipSuccess &&
               hipMalloc(&c->rec, cap * deposit_record_bytes(f32)) == hipSuccess &&
-              hipMalloc(&c->rec_sorted, cap * deposit_record_bytes(f32)) == hipSuccess &&
+              hipMalloc(&c->pid, cap * sizeof(int)) == hipSuccess &&
+              hipMalloc(&c->pos, cap * sizeof(int)) == hipSuccess &&
               hipMalloc(&c->big, dense_cell_capacity(cap) * sizeof(int)) == hipSuccess &&
               hipMalloc(&c->nbig, sizeof(int)) == hipSuccess &&
               hipMalloc(&c->dslot, nn * sizeof(int)) == hipSuccess &&
@@ -595,7 +597,8 @@ void lc3d_destroy(lc3d_ctx *c)
     (void)hipFree(c->idx);
     (void)hipFree(c->err);
     (void)hipFree(c->rec);
-    (void)hipFree(c->rec_sorted);
+    (void)hipFree(c->pid);
+    (void)hipFree(c->pos);
     (void)hipFree(c->big);
     (void)hipFree(c->nbig);
     (void)hipFree(c->dslot);

@@ -122,13 +122,13 @@ __device__ __forceinline__ int count_rank(int *count, int key)
 }
 
 // Bins a plasma particle for the deposit: returns its cell key ix N + iy and
-// writes its record at rec[k], or returns -1 (with an error bit) when the 5x5
+// writes its record at rec[j] (its binning position), or returns -1 (with an error bit) when the 5x5
 // stencil would leave the grid or the position is not finite.  P Q Z M qv are
 // the stored px py pz m q.
 template <typename TP>
 __device__ __forceinline__ int bin_particle(double x, double y, double h, int N, double P, double Q,
                                             double Z, double M, double qv, DepRec<TP> *rec,
-                                            int64_t k, unsigned *err)
+                                            int64_t j, unsigned *err)
 {
     int ix, iy;
     double xl, yl;
@@ -148,7 +148,7 @@ __device__ __forceinline__ int bin_particle(double x, double y, double h, int N,
     r.c2 = (TP)(Q * r_g);
     r.c3 = (TP)(Z * r_g);
     r.iy = iy;
-    rec[k] = r;
+    rec[j] = r;
     return ix * N + iy;
 }
 

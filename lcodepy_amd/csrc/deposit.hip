@@ -5,27 +5,32 @@
 // gather over cell-sorted particles:
 //
 //   1. bin      cell of every particle (fused into the push when possible),
+//               written with its deposit record at its binning position
+//               (the binning thread's index, so the writes are contiguous),
 //               integer atomic count per cell (one atomic per run of equal
 //               cells in a wave), which also gives each particle its rank in
 //               its cell;
 //   2. scan     exclusive prefix of the counts -> start of each cell's list
 //               (the counts are cleared for the next deposit);
-//   3. scatter  particle ids into their cell's list at their ranks;
-//   4. order    each cell's list sorted by particle id (by the cell's thread,
-//               or for a dense cell by a workgroup in step 5), so the
-//               summation order below does not depend on atomic arrival
-//               order;
+//   3. scatter  binning positions into their cell's list at their ranks;
+//   4. order    each cell's list sorted by particle id (by the cell's
+//               thread, or for a dense cell by a workgroup in step 5), so
+//               the summation order below depends neither on atomic arrival
+//               order nor on the order the particles were visited in; the
+//               particle ids of the lists are the cell order the next push
+//               visits (so binning positions follow the cell order, and a
+//               cell's records lie close together);
 //   5. taps     per dense cell (more than 16 particles) its list sorted and
 //               its 5 x 5 x 4 taps, by one workgroup with a fixed-order
 //               reduction;
-//   6. gather   one workgroup per 16x16 tile of output cells forms, in five
-//               passes over the x offset, the taps of the 20x20 source
-//               cells around it in LDS; each thread adds, for its own output
-//               cell, the 25 taps that reach it in a fixed order.
+//   6. gather   one workgroup per 16x16 tile of output cells forms the taps
+//               of the 20x20 source cells around it (records read through
+//               the lists), one x offset at a time in LDS; each output cell
+//               adds the 25 taps that reach it in a fixed order.
 //
-// Every output cell is written by exactly one thread: no floating-point
-// atomics, and the result is bit-reproducible run to run.  In the FP32-bulk
-// form the tile accumulators are Kahan-compensated.
+// Every output value is written by exactly one thread: no floating-point
+// atomics, and the result is bit-reproducible run to run.  The tile sums are
+// float64 in both forms.
 #include "common.h"
 #include "kernels.h"
 
@@ -54,6 +59,7 @@ __global__ __launch_bounds__(256) void k_bin(int64_t n, double h, int N,
                                              const TP *__restrict__ m, BinState bs)
 {
     // Threads past the end stay to the count so that whole waves reach it.
+    // Particle k is binned at position k.
     const int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     int key = -1;
     if (k < n) {
@@ -62,6 +68,7 @@ __global__ __launch_bounds__(256) void k_bin(int64_t n, double h, int N,
                                    (double)py[k], (double)pz[k], (double)m[k], (double)q[k],
                                    static_cast<DepRec<TP> *>(bs.rec), k, bs.err);
         bs.key[k] = key;
+        bs.pid[k] = (int)k;
     }
     // Unbinned particles are counted (and ranked) in count[N * N].
     const int r = count_rank(bs.count, k < n ? (key >= 0 ? key : N * N) : -1);
@@ -135,74 +142,105 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_scan_final(int ncell, int nblo
     (void)nblocks;
 }
 
-// ---- scatter ids into the cell lists --------------------------------------------
-// Places particle k at its rank in its cell's list; particles that are not
-// binned (q = 0 or off the grid) go after all the lists, at their rank among
-// the unbinned ones, so that idx is a permutation of the sort.  count[ncell]
-// counts them and is cleared by k_order_cells.
+// ---- scatter positions into the cell lists ---------------------------------------
+// Places binning position j, and the particle binned there, at its rank in
+// its cell's list; particles that are not binned (q = 0 or off the grid) go
+// after all the lists, at their rank among the unbinned ones, so that pos and
+// idx are permutations.  count[ncell] counts them and is cleared by
+// k_order_cells.
 __global__ __launch_bounds__(256) void k_scatter(int64_t n, int ncell, BinState bs)
 {
-    const int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    if (k >= n) return;
-    if (k == 0) *bs.nbig = 0;   // k_order_cells refills the big-cell list
-    const int key = bs.key[k];
-    bs.idx[bs.start[key < 0 ? ncell : key] + bs.rank[k]] = (int)k;
+    const int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    if (j == 0) *bs.nbig = 0;   // k_order_cells refills the big-cell list
+    const int key = bs.key[j];
+    const int slot = bs.start[key < 0 ? ncell : key] + bs.rank[j];
+    bs.pos[slot] = (int)j;
+    bs.idx[slot] = bs.pid[j];
 }
 
-// ---- sort each cell's list by particle id ---------------------------------------
-// Lists of up to SMALL_CELL particles are sorted by their cell's thread
-// (insertion sort).  Longer ones make the cell dense: it is queued (its slot
-// recorded in dslot) for k_dense_cells, which sorts it.
+// ---- sort each cell's list by particle id ------------------------------------------
+// Insertion sort of list [s, e) by particle id (idx), pos alongside, in memory.
+__device__ void sort_list_in_memory(const BinState &bs, int s, int e)
+{
+    int *a = bs.pos, *d = bs.idx;
+    for (int i = s + 1; i < e; ++i) {
+        const int v = d[i], p = a[i];
+        int j = i - 1;
+        while (j >= s && d[j] > v) {
+            d[j + 1] = d[j];
+            a[j + 1] = a[j];
+            --j;
+        }
+        d[j + 1] = v;
+        a[j + 1] = p;
+    }
+}
+
+// One thread per cell.  A list of up to SMALL_CELL particles has its
+// particle ids read into registers and, if they are out of order (the usual
+// list, whose particles were visited in the last deposit's order, is in order
+// unless a particle arrived from another wave), is sorted in this thread's
+// row of LDS and written back, pos alongside.  A longer list makes the cell
+// dense: it is queued (its slot recorded in dslot) for k_dense_cells, which
+// sorts it, or, when the queue is full, sorted here in memory.
 __global__ __launch_bounds__(256) void k_order_cells(int ncell, BinState bs)
 {
+    __shared__ int s_d[256][SMALL_CELL + 1], s_a[256][SMALL_CELL + 1];
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c == 0) bs.count[ncell] = 0;   // the scatter's count of unbinned particles
     if (c >= ncell) return;
-    const int s = bs.start[c], e = bs.start[c + 1];
+    const int s = bs.start[c], e = bs.start[c + 1], n = e - s;
     int slot = -1;
-    if (e - s > SMALL_CELL) {
+    if (n > SMALL_CELL) {
         // At most n / (SMALL_CELL + 1) < bigcap cells can be dense.
         slot = atomicAdd(bs.nbig, 1);
         if (slot < bs.bigcap) bs.big[slot] = c;
         else slot = -1;
-        if (e - s > MAX_SORTED_CELL) atomicOr(bs.err, ERR_CELL_OVERFLOW);
+        if (n > MAX_SORTED_CELL) atomicOr(bs.err, ERR_CELL_OVERFLOW);
+        if (slot < 0) sort_list_in_memory(bs, s, e);
     }
     bs.dslot[c] = slot;
-    if (slot >= 0) return;
-    int *a = bs.idx;
-    for (int i = s + 1; i < e; ++i) {
-        const int v = a[i];
+    if (n > SMALL_CELL) return;
+    int d[SMALL_CELL];
+#pragma unroll
+    for (int k = 0; k < SMALL_CELL; ++k) d[k] = k < n ? bs.idx[s + k] : INT_MAX;
+    bool sorted = true;
+#pragma unroll
+    for (int k = 1; k < SMALL_CELL; ++k)
+        if (d[k - 1] > d[k]) sorted = false;   // the INT_MAX padding stays in order
+    if (sorted) return;
+    int *dd = s_d[threadIdx.x], *aa = s_a[threadIdx.x];
+#pragma unroll
+    for (int k = 0; k < SMALL_CELL; ++k) {
+        if (k < n) {
+            dd[k] = d[k];
+            aa[k] = bs.pos[s + k];
+        }
+    }
+    for (int i = 1; i < n; ++i) {
+        const int v = dd[i], p = aa[i];
         int j = i - 1;
-        while (j >= s && a[j] > v) {
-            a[j + 1] = a[j];
+        while (j >= 0 && dd[j] > v) {
+            dd[j + 1] = dd[j];
+            aa[j + 1] = aa[j];
             --j;
         }
-        a[j + 1] = v;
+        dd[j + 1] = v;
+        aa[j + 1] = p;
     }
-}
-
-// The deposit records in list order: rec_sorted[i] = rec[idx[i]] for every
-// binned list position i < start[ncell], one thread per 16-byte word of a
-// record (a record is rec_words words), so the writes are contiguous.  The
-// gather reads the non-dense cells' records from here.
-__global__ __launch_bounds__(256) void k_sort_records(int ncell, BinState bs, int rec_words)
-{
-    const int64_t nwords = (int64_t)bs.start[ncell] * rec_words;
-    const int4 *__restrict__ src = static_cast<const int4 *>(bs.rec);
-    int4 *__restrict__ dst = static_cast<int4 *>(bs.rec_sorted);
-    for (int64_t o = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; o < nwords;
-         o += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t i = o / rec_words;
-        const int w = (int)(o - i * rec_words);
-        dst[o] = src[(int64_t)bs.idx[i] * rec_words + w];
+    for (int k = 0; k < n; ++k) {
+        bs.idx[s + k] = dd[k];
+        bs.pos[s + k] = aa[k];
     }
 }
 
 // ---- dense cells: sort and taps ------------------------------------------------------
 // One workgroup per dense cell, the grid striding over the queue.  The cell's
-// list is sorted by particle id with a bitonic sort in LDS (lists of up to
-// MAX_SORTED_CELL; longer ones keep arrival order) and written back, so the
-// next push visits the cell's particles in id order.  Then thread t sums its
+// list is sorted by particle id with a bitonic sort of (particle id,
+// position) keys in LDS (lists of up to MAX_SORTED_CELL; longer ones keep
+// arrival order) and written back, positions to pos and particle ids to idx
+// (the next push's order).  Then thread t sums its
 // contiguous segment of the sorted list into the 5 x 5 taps x 4 currents
 // around the cell; the 256 partial sums are combined by a butterfly within
 // each wave and then the four waves in a fixed order.  out[slot][tap][q],
@@ -212,7 +250,7 @@ constexpr int DENSE_THREADS = 256;
 template <typename T>
 __global__ __launch_bounds__(DENSE_THREADS) void k_dense_cells(double h, BinState bs)
 {
-    __shared__ int s[MAX_SORTED_CELL];
+    __shared__ unsigned long long s[MAX_SORTED_CELL];   // particle id << 32 | position
     __shared__ T red[4][DENSE_VALUES];
     const int nbig = min(*bs.nbig, bs.bigcap);
     const DepRec<T> *__restrict__ rec = static_cast<const DepRec<T> *>(bs.rec);
@@ -226,14 +264,19 @@ __global__ __launch_bounds__(DENSE_THREADS) void k_dense_cells(double h, BinStat
         if (sorted) {
             int P = 1;
             while (P < n) P <<= 1;
-            for (int i = t; i < P; i += DENSE_THREADS) s[i] = i < n ? bs.idx[st + i] : INT_MAX;
+            for (int i = t; i < P; i += DENSE_THREADS) {
+                unsigned long long v = ~0ull;
+                if (i < n)
+                    v = (unsigned long long)(unsigned)bs.idx[st + i] << 32 | (unsigned)bs.pos[st + i];
+                s[i] = v;
+            }
             __syncthreads();
             for (int k = 2; k <= P; k <<= 1) {
                 for (int j = k >> 1; j > 0; j >>= 1) {
                     for (int i = t; i < P; i += DENSE_THREADS) {
                         const int l = i ^ j;
                         if (l > i) {
-                            const int x = s[i], y = s[l];
+                            const unsigned long long x = s[i], y = s[l];
                             if ((x > y) == ((i & k) == 0)) {
                                 s[i] = y;
                                 s[l] = x;
@@ -243,7 +286,10 @@ __global__ __launch_bounds__(DENSE_THREADS) void k_dense_cells(double h, BinStat
                     __syncthreads();
                 }
             }
-            for (int i = t; i < n; i += DENSE_THREADS) bs.idx[st + i] = s[i];
+            for (int i = t; i < n; i += DENSE_THREADS) {
+                bs.pos[st + i] = (int)(unsigned)s[i];
+                bs.idx[st + i] = (int)(s[i] >> 32);
+            }
         }
         const int seg = (n + DENSE_THREADS - 1) / DENSE_THREADS;
         const int j0 = min(n, t * seg), j1 = min(n, (t + 1) * seg);
@@ -251,7 +297,7 @@ __global__ __launch_bounds__(DENSE_THREADS) void k_dense_cells(double h, BinStat
 #pragma unroll
         for (int k = 0; k < DENSE_VALUES; ++k) acc[k] = T(0);
         for (int j = j0; j < j1; ++j) {
-            const DepRec<T> d = rec[sorted ? s[j] : bs.idx[st + j]];
+            const DepRec<T> d = rec[sorted ? (int)(unsigned)s[j] : bs.pos[st + j]];
             T wx[5], wy[5];
             weights4<T>(d.xl, wx);
             weights4<T>(d.yl, wy);
@@ -284,70 +330,65 @@ __global__ __launch_bounds__(DENSE_THREADS) void k_dense_cells(double h, BinStat
 }
 
 // ---- gather deposit -------------------------------------------------------------------
-template <typename T, bool KAHAN> struct Acc;
-template <typename T> struct Acc<T, false> {
-    T s = 0;
-    __device__ __forceinline__ void add(T v) { s += v; }
-    __device__ __forceinline__ T get() const { return s; }
-};
-template <typename T> struct Acc<T, true> {
-    T s = 0, c = 0;
-    __device__ __forceinline__ void add(T v) {
-        const T y = v - c;
-        const T t = s + y;
-        c = (t - s) - y;
-        s = t;
-    }
-    __device__ __forceinline__ T get() const { return s; }
-};
-
 template <typename T> struct alignas(4 * sizeof(T)) Tap4 {
     T v[4];
 };
 
-// The five taps toward x offset A - 2 (target cells (ix + A - 2, iy + b - 2),
-// b = 0..4) of one source cell, for the four currents: the sum over its
-// particles, in list order, of c_q (w_A(xl) / h) (w_b(yl) / h), from the
-// records in list order (rec = rec_sorted, first = the cell's list start),
-// or the precomputed block of a dense cell.
-// Records of a halo cell's first particles kept in registers for all five
-// passes of the gather.
-template <typename T> struct TapPrefetch {
-    static constexpr int PF = sizeof(T) == 4 ? 4 : 2;
+// One block per 16 x 16 tile of output cells.  Its 20 x 20 halo of source
+// cells is dealt to the block's threads one cell each, ordered by particle
+// count (so that the cells of a wave have about as many particles and the
+// wave's loop over them wastes few lanes); a thread forms its cell's taps,
+// group by group of x offsets, holding a group's taps in registers while it
+// reads the cell's records once for the whole group.  Tap groups (number of
+// x offsets A per group) and the waves per SIMD asked of the compiler
+// (__launch_bounds__'s second argument, which bounds the registers):
+// float32 taps in groups of 3 and 2 (60 values in registers), float64 one x
+// offset at a time in a runtime loop (fewer registers, two blocks per CU);
+// rf records of a cell in flight at a time.  The positions of each halo
+// cell's first POS_SLOTS particles are read once into LDS, so that a group's
+// record loads wait for one global read, not two.
+constexpr int GATHER_THREADS = 512;
+constexpr int HALO_CELLS = DEP_HALO * DEP_HALO;
+static_assert(HALO_CELLS <= GATHER_THREADS, "one halo cell per thread");
+static_assert(GATHER_THREADS == 2 * DEP_TILE * DEP_TILE, "two threads per output cell");
+constexpr int COST_CLASSES = SMALL_CELL + 2;   // 0 .. SMALL_CELL particles, more
+constexpr int POS_SLOTS = 8;
+template <typename T> struct GatherPlan;
+template <> struct GatherPlan<float> {
+    static constexpr int g0 = 3, g1 = 2, g2 = 0, g3 = 0, g4 = 0, rf = 4, waves = 4;
+    static constexpr bool rt = false;
 };
-// Gather blocks per CU asked of the compiler: five in float32, whose 31 KB of
-// LDS taps allow it, so that the 33 x 33 tiles of a 513 x 513 grid run in one
-// round; two in float64 (62 KB).
-template <typename T> struct GatherOcc {
-    static constexpr int value = sizeof(T) == 4 ? 5 : 2;
+template <> struct GatherPlan<double> {
+    static constexpr int g0 = 1, g1 = 1, g2 = 1, g3 = 1, g4 = 1, rf = 2, waves = 4;
+    static constexpr bool rt = true;   // one x offset at a time, in a runtime loop
 };
 
-template <int A, typename T>
-__device__ __forceinline__ void add_taps(const DepRec<T> &d, T inv_h, Tap4<T> tp[5])
+// Binning position of particle k of a cell's list: from the cell's LDS row
+// (spos) for its first POS_SLOTS particles, else from pos.
+__device__ __forceinline__ int list_pos(const int *spos, const int *__restrict__ pos, int first,
+                                        int k)
 {
-    const T wx = w4_tap<A>(d.xl) * inv_h;
-    T wy[5];
-    weights4<T>(d.yl, wy);
-#pragma unroll
-    for (int b = 0; b < 5; ++b) {
-        const T w = wx * (wy[b] * inv_h);
-        tp[b].v[0] += d.c0 * w;
-        tp[b].v[1] += d.c1 * w;
-        tp[b].v[2] += d.c2 * w;
-        tp[b].v[3] += d.c3 * w;
+    return k < POS_SLOTS ? spos[k] : pos[first + k];
+}
+
+// Tap A (0..4) of weights4 for a wave-uniform A (a scalar branch).
+template <typename T> __device__ __forceinline__ T w4_uniform(int A, T x)
+{
+    switch (A) {
+    case 0: return w4_m2(x);
+    case 1: return w4_m1(x);
+    case 2: return w4_0(x);
+    case 3: return w4_p1(x);
+    default: return w4_p2(x);
     }
 }
 
-// The five taps toward x offset A - 2 (target cells (ix + A - 2, iy + b - 2),
-// b = 0..4) of one source cell, for the four currents: the sum over its
-// particles, in list order, of c_q (w_A(xl) / h) (w_b(yl) / h) -- the first
-// PF (0 or TapPrefetch) records from the registers `pre`, the others from rec
-// (= rec_sorted, first = the cell's list start) -- or the precomputed block
-// of a dense cell.
-template <int A, typename T, int PF>
-__device__ __forceinline__ void cell_taps(const DepRec<T> *__restrict__ rec,
-                                          const T *__restrict__ dense, int first, int n, int dsl,
-                                          const DepRec<T> *pre, T inv_h, Tap4<T> tp[5])
+// cell_taps for the single, wave-uniform x offset A.
+template <typename T>
+__device__ __forceinline__ void cell_taps_uniform(int A, const DepRec<T> *__restrict__ rec,
+                                                  const int *spos, const int *__restrict__ pos,
+                                                  const T *__restrict__ dense, int first, int n,
+                                                  int dsl, T inv_h, Tap4<T> (&tp)[5])
 {
 #pragma unroll
     for (int b = 0; b < 5; ++b)
@@ -361,133 +402,268 @@ __device__ __forceinline__ void cell_taps(const DepRec<T> *__restrict__ rec,
             for (int q = 0; q < 4; ++q) tp[b].v[q] = v[4 * b + q];
         return;
     }
+    constexpr int RF = GatherPlan<T>::rf;
+    for (int j0 = 0; j0 < n; j0 += RF) {
+        DepRec<T> rk[RF];
 #pragma unroll
-    for (int k = 0; k < PF; ++k)
-        if (k < n) add_taps<A, T>(pre[k], inv_h, tp);
-    // The rest, up to four records in flight at a time.
-    for (int j0 = PF; j0 < n; j0 += 4) {
-        DepRec<T> d[4];
+        for (int k = 0; k < RF; ++k)
+            if (j0 + k < n) rk[k] = rec[list_pos(spos, pos, first, j0 + k)];
 #pragma unroll
-        for (int k = 0; k < 4; ++k)
-            if (j0 + k < n) d[k] = rec[first + j0 + k];
-#pragma unroll
-        for (int k = 0; k < 4; ++k) {
+        for (int k = 0; k < RF; ++k) {
             if (j0 + k >= n) break;
-            add_taps<A, T>(d[k], inv_h, tp);
+            const DepRec<T> &d = rk[k];
+            // The weight product first, then the currents (plasma3d/weights.py:58-67).
+            T wy[5];
+            weights4<T>(d.yl, wy);
+            const T wx = w4_uniform<T>(A, d.xl) * inv_h;
+#pragma unroll
+            for (int b = 0; b < 5; ++b) {
+                const T w = wx * (wy[b] * inv_h);
+                tp[b].v[0] += d.c0 * w;
+                tp[b].v[1] += d.c1 * w;
+                tp[b].v[2] += d.c2 * w;
+                tp[b].v[3] += d.c3 * w;
+            }
         }
     }
 }
 
-// One block per 16 x 16 tile of output cells, one thread per output cell,
-// in five passes over the x offset A of the source cells (source row
-// ox - (A - 2)).  In a pass every cell of the tile's 20 x 20 halo forms its
-// five taps toward that offset (cell_taps: from the deposit records its
-// particles' binning pass wrote, in list order, or a dense cell's
-// precomputed block) into LDS; then each output cell adds the five taps that
-// reach it (source column oy - (b - 2), b ascending).  Every output cell is
-// thus the same fixed-order sum of 25 taps, written by exactly one thread:
-// no floating-point atomics, reproducible bit for bit.
-template <typename TP, typename TG, bool KAHAN>
-__global__ __launch_bounds__(256, GatherOcc<TP>::value) void k_deposit_gather(
+// Adds one particle's taps toward x offsets A0 .. A0 + G - 1 (target cells
+// (ix + A - 2, iy + b - 2)): tp[A - A0][b].v[q] += c_q (w_A(xl) / h)
+// (w_b(yl) / h).
+template <int A0, int G, typename T>
+__device__ __forceinline__ void add_taps(const DepRec<T> &d, T inv_h, Tap4<T> (&tp)[G][5])
+{
+    T wx[5], wy[5];
+    weights4<T>(d.xl, wx);   // the taps outside the group are dropped by the compiler
+    weights4<T>(d.yl, wy);
+#pragma unroll
+    for (int b = 0; b < 5; ++b) wy[b] *= inv_h;
+#pragma unroll
+    for (int g = 0; g < G; ++g) {
+        const T w = wx[A0 + g] * inv_h;
+        const T c0 = d.c0 * w, c1 = d.c1 * w, c2 = d.c2 * w, c3 = d.c3 * w;
+#pragma unroll
+        for (int b = 0; b < 5; ++b) {
+            tp[g][b].v[0] += c0 * wy[b];
+            tp[g][b].v[1] += c1 * wy[b];
+            tp[g][b].v[2] += c2 * wy[b];
+            tp[g][b].v[3] += c3 * wy[b];
+        }
+    }
+}
+
+// The taps of one source cell toward x offsets A0 .. A0 + G - 1: the sum
+// over its particles, in list order, of the records the list names (pos,
+// first = the cell's list start), or the precomputed block of a dense cell
+// (dsl >= 0).
+template <int A0, int G, typename T>
+__device__ __forceinline__ void cell_taps(const DepRec<T> *__restrict__ rec, const int *spos,
+                                          const int *__restrict__ pos,
+                                          const T *__restrict__ dense, int first, int n, int dsl,
+                                          T inv_h, Tap4<T> (&tp)[G][5])
+{
+#pragma unroll
+    for (int g = 0; g < G; ++g)
+#pragma unroll
+        for (int b = 0; b < 5; ++b)
+#pragma unroll
+            for (int q = 0; q < 4; ++q) tp[g][b].v[q] = T(0);
+    if (dsl >= 0) {
+        const T *v = dense + (size_t)dsl * DENSE_VALUES + 4 * (5 * A0);
+#pragma unroll
+        for (int g = 0; g < G; ++g)
+#pragma unroll
+            for (int b = 0; b < 5; ++b)
+#pragma unroll
+                for (int q = 0; q < 4; ++q) tp[g][b].v[q] = v[20 * g + 4 * b + q];
+        return;
+    }
+    // Up to RF records in flight at a time.
+    constexpr int RF = GatherPlan<T>::rf;
+    for (int j0 = 0; j0 < n; j0 += RF) {
+        DepRec<T> d[RF];
+#pragma unroll
+        for (int k = 0; k < RF; ++k)
+            if (j0 + k < n) d[k] = rec[list_pos(spos, pos, first, j0 + k)];
+#pragma unroll
+        for (int k = 0; k < RF; ++k) {
+            if (j0 + k >= n) break;
+            add_taps<A0, G, T>(d[k], inv_h, tp);
+        }
+    }
+}
+
+template <typename T> struct alignas(2 * sizeof(T)) Tap2 {
+    T v[2];
+};
+
+// Every output cell is the fixed-order sum (x offset A ascending, then y
+// offset b ascending) of the 25 taps that reach it, added in float64 by the
+// two threads that own its four currents (thread t: cell t % 256, currents
+// 2 (t / 256) and 2 (t / 256) + 1), and written by them alone: no
+// floating-point atomics, reproducible bit for bit.  The LDS tap tile always
+// spans the full 20 x 20 halo; its cells outside the grid hold zero taps, so
+// that the reads need no bounds checks.
+template <typename TP, typename TG>
+__global__ __launch_bounds__(GATHER_THREADS, GatherPlan<TP>::waves) void k_deposit_gather(
     int N, double h, const int *__restrict__ start, BinState bs, TG *__restrict__ ro,
     TG *__restrict__ jx, TG *__restrict__ jy, TG *__restrict__ jz, const TG *__restrict__ bg_ro,
     TG *__restrict__ ro1, TG *__restrict__ jx1, TG *__restrict__ jy1, TG *__restrict__ jz1)
 {
     using T = TP;   // arithmetic type of the taps
+    using P = GatherPlan<T>;
+    static_assert(P::g0 + P::g1 + P::g2 + P::g3 + P::g4 == 5, "five x offsets");
     constexpr int H = DEP_HALO;
-    __shared__ Tap4<T> s_tap[5][H * H];
+    // One x offset's taps per halo cell, currents (ro, jx) and (jy, jz) apart.
+    __shared__ Tap2<T> s_tap[2][5][HALO_CELLS];
+    // List start, and particle count n or, for a dense cell, -1 - its slot.
+    __shared__ int s_first[HALO_CELLS], s_nd[HALO_CELLS];
+    __shared__ int s_pos[HALO_CELLS][POS_SLOTS];   // binning positions of its first particles
+    __shared__ short s_cell[HALO_CELLS];       // halo cells by particle count
+    __shared__ int s_class[COST_CLASSES];
 
     const int tile = xcd_swizzle(blockIdx.x, gridDim.x);
     const int tpr = (N + DEP_TILE - 1) / DEP_TILE;
     const int X0 = (tile / tpr) * DEP_TILE, Y0 = (tile % tpr) * DEP_TILE;
-    const DepRec<T> *__restrict__ rec = static_cast<const DepRec<T> *>(bs.rec_sorted);
+    const DepRec<T> *__restrict__ rec = static_cast<const DepRec<T> *>(bs.rec);
     const T *__restrict__ dense = static_cast<const T *>(bs.dense);
-    const int hx0 = max(0, X0 - 2), hx1 = min(N, X0 + DEP_TILE + 2);
-    const int hy0 = max(0, Y0 - 2), hy1 = min(N, Y0 + DEP_TILE + 2);
-    const int nrow = hx1 - hx0, ncol = hy1 - hy0, nc = nrow * ncol;
     const int t = threadIdx.x;
 
-    // This thread's halo cells, t and t + 256 (nc <= 400): list start,
-    // particle count and dense slot (or -1).
-    int c_first[2] = {0, 0}, c_cnt[2] = {0, 0}, c_dsl[2] = {-1, -1};
+    // Halo cell t (row t / 20, column t % 20 from (X0 - 2, Y0 - 2)): list
+    // start, particle count, dense slot (or -1); none for a cell outside the
+    // grid.  Then the cells ordered by cost class (particle count, a dense
+    // cell counting as one) by a counting sort in LDS.  The order within a
+    // class is arrival order, which decides only which thread forms which
+    // cell's taps.
+    if (t < COST_CLASSES) s_class[t] = 0;
+    __syncthreads();
+    int cls = 0, in_cls = 0;
+    if (t < HALO_CELLS) {
+        const int cx = X0 - 2 + t / H, cy = Y0 - 2 + t % H;
+        int first = 0, n = 0, dsl = -1;
+        if (cx >= 0 && cx < N && cy >= 0 && cy < N) {
+            const int cell = cx * N + cy;
+            first = start[cell];
+            n = start[cell + 1] - first;
+            dsl = bs.dslot[cell];
+        }
+        s_first[t] = first;
+        s_nd[t] = dsl >= 0 ? -1 - dsl : n;
+        if (dsl < 0) {
 #pragma unroll
-    for (int u = 0; u < 2; ++u) {
-        const int i = t + 256 * u;
-        if (i < nc) {
-            const int cell = (hx0 + i / ncol) * N + hy0 + i % ncol;
-            c_first[u] = start[cell];
-            c_cnt[u] = start[cell + 1] - c_first[u];
-            c_dsl[u] = bs.dslot[cell];
+            for (int k = 0; k < POS_SLOTS; ++k)
+                if (k < n) s_pos[t][k] = bs.pos[first + k];
+        }
+        cls = dsl >= 0 ? 1 : min(n, SMALL_CELL + 1);
+        in_cls = atomicAdd(&s_class[cls], 1);
+    }
+    __syncthreads();
+    if (t == 0) {
+        int run = 0;
+        for (int c = 0; c < COST_CLASSES; ++c) {
+            const int k = s_class[c];
+            s_class[c] = run;
+            run += k;
         }
     }
+    __syncthreads();
+    if (t < HALO_CELLS) s_cell[s_class[cls] + in_cls] = (short)t;
+    __syncthreads();
+    const int mine = t < HALO_CELLS ? s_cell[t] : -1;
+    const int m_first = mine >= 0 ? s_first[mine] : 0;
+    const int m_nd = mine >= 0 ? s_nd[mine] : 0;
+    const int m_n = m_nd >= 0 ? m_nd : 0;
+    const int m_dsl = m_nd < 0 ? -1 - m_nd : -1;
 
-    const int tx = t / DEP_TILE, ty = t % DEP_TILE;
+    const int o = t % (DEP_TILE * DEP_TILE), pair = t / (DEP_TILE * DEP_TILE);
+    const int tx = o / DEP_TILE, ty = o % DEP_TILE;
     const int ox = X0 + tx, oy = Y0 + ty;
-    const bool valid = ox < N && oy < N;
+    // Source cell (ox - (A - 2), oy - (b - 2)) sits at halo position
+    // base + (4 - A) H + (4 - b).
+    const int base = tx * H + ty;
     const T inv_h = T(1.0 / h);
-    Acc<T, KAHAN> acc[4];
-    static_assert(DEP_HALO * DEP_HALO <= 512, "two halo cells per thread");
+    double acc0 = 0, acc1 = 0;   // currents 2 pair and 2 pair + 1
 
-    // This thread's halo cells (t and t + 256; nc <= 400) and their first
-    // records, loaded once for the five passes.
-    // (Only the first cell's: a thread's second cell, t + 256 < nc, reads its
-    // records in each pass.)
-    constexpr int PF = TapPrefetch<T>::PF;
-    DepRec<T> pre[PF];
-    if (t < nc && c_dsl[0] < 0) {
+    auto group = [&](auto a0_const, auto g_const) {
+        constexpr int A0 = decltype(a0_const)::value, G = decltype(g_const)::value;
+        Tap4<T> tp[G][5];
+        if (mine >= 0)
+            cell_taps<A0, G, T>(rec, s_pos[mine], bs.pos, dense, m_first, m_n, m_dsl, inv_h, tp);
 #pragma unroll
-        for (int k = 0; k < PF; ++k)
-            if (k < c_cnt[0]) pre[k] = rec[c_first[0] + k];
-    }
-
-    auto pass = [&](auto a_const) {
-        constexpr int A = decltype(a_const)::value;
+        for (int g = 0; g < G; ++g) {
+            if (mine >= 0) {
 #pragma unroll
-        for (int u = 0; u < 2; ++u) {
-            const int i = t + 256 * u;
-            if (i < nc) {
-                Tap4<T> tp[5];
-                if (u == 0)
-                    cell_taps<A, T, TapPrefetch<T>::PF>(rec, dense, c_first[0], c_cnt[0], c_dsl[0],
-                                                        pre, inv_h, tp);
-                else
-                    cell_taps<A, T, 0>(rec, dense, c_first[1], c_cnt[1], c_dsl[1], pre, inv_h, tp);
-#pragma unroll
-                for (int b = 0; b < 5; ++b) s_tap[b][i] = tp[b];
-            }
-        }
-        __syncthreads();
-        const int r = ox - (A - 2) - hx0;
-        if (valid && r >= 0 && r < nrow) {
-#pragma unroll
-            for (int b = 0; b < 5; ++b) {
-                const int c = oy - (b - 2) - hy0;
-                if (c >= 0 && c < ncol) {
-                    const Tap4<T> v = s_tap[b][r * ncol + c];
-#pragma unroll
-                    for (int q = 0; q < 4; ++q) acc[q].add(v.v[q]);
+                for (int b = 0; b < 5; ++b) {
+                    s_tap[0][b][mine] = Tap2<T>{{tp[g][b].v[0], tp[g][b].v[1]}};
+                    s_tap[1][b][mine] = Tap2<T>{{tp[g][b].v[2], tp[g][b].v[3]}};
                 }
             }
+            __syncthreads();
+            const Tap2<T> *row = &s_tap[pair][0][base + (4 - (A0 + g)) * H];
+#pragma unroll
+            for (int b = 0; b < 5; ++b) {
+                const Tap2<T> v = row[b * HALO_CELLS + 4 - b];
+                acc0 += (double)v.v[0];
+                acc1 += (double)v.v[1];
+            }
+            __syncthreads();
         }
-        __syncthreads();
     };
-    pass(std::integral_constant<int, 0>{});
-    pass(std::integral_constant<int, 1>{});
-    pass(std::integral_constant<int, 2>{});
-    pass(std::integral_constant<int, 3>{});
-    pass(std::integral_constant<int, 4>{});
+    if constexpr (P::rt) {
+#pragma unroll 1
+        for (int A = 0; A < 5; ++A) {
+            Tap4<T> tp[5];
+            if (mine >= 0)
+                cell_taps_uniform<T>(A, rec, s_pos[mine], bs.pos, dense, m_first, m_n, m_dsl,
+                                     inv_h, tp);
+            if (mine >= 0) {
+#pragma unroll
+                for (int b = 0; b < 5; ++b) {
+                    s_tap[0][b][mine] = Tap2<T>{{tp[b].v[0], tp[b].v[1]}};
+                    s_tap[1][b][mine] = Tap2<T>{{tp[b].v[2], tp[b].v[3]}};
+                }
+            }
+            __syncthreads();
+            const Tap2<T> *row = &s_tap[pair][0][base + (4 - A) * H];
+#pragma unroll
+            for (int b = 0; b < 5; ++b) {
+                const Tap2<T> v = row[b * HALO_CELLS + 4 - b];
+                acc0 += (double)v.v[0];
+                acc1 += (double)v.v[1];
+            }
+            __syncthreads();
+        }
+    } else {
+    group(std::integral_constant<int, 0>{}, std::integral_constant<int, P::g0>{});
+    group(std::integral_constant<int, P::g0>{}, std::integral_constant<int, P::g1>{});
+    if constexpr (P::g2 > 0)
+        group(std::integral_constant<int, P::g0 + P::g1>{}, std::integral_constant<int, P::g2>{});
+    if constexpr (P::g3 > 0)
+        group(std::integral_constant<int, P::g0 + P::g1 + P::g2>{},
+              std::integral_constant<int, P::g3>{});
+    if constexpr (P::g4 > 0)
+        group(std::integral_constant<int, P::g0 + P::g1 + P::g2 + P::g3>{},
+              std::integral_constant<int, P::g4>{});
+    }
 
-    if (valid) {
-        const int o = ox * N + oy;
-        ro[o] = (TG)acc[0].get();
-        jx[o] = (TG)acc[1].get();
-        jy[o] = (TG)acc[2].get();
-        jz[o] = (TG)acc[3].get();
+    if (ox < N && oy < N) {
+        const int oc = ox * N + oy;
+        if (pair == 0) {
+            ro[oc] = (TG)acc0;
+            jx[oc] = (TG)acc1;
+        } else {
+            jy[oc] = (TG)acc0;
+            jz[oc] = (TG)acc1;
+        }
         if (bg_ro != nullptr) {
-            ro1[o] = bg_ro[o];
-            jx1[o] = 0;
-            jy1[o] = 0;
-            jz1[o] = 0;
+            if (pair == 0) {
+                ro1[oc] = bg_ro[oc];
+                jx1[oc] = 0;
+            } else {
+                jy1[oc] = 0;
+                jz1[oc] = 0;
+            }
         }
     }
 }
@@ -506,7 +682,6 @@ hipError_t launch_deposit(const PushGeom &g, const SortArrays<TP> &a, bool use_f
     const double *pz = use_full ? a.pz : a.pz_prev;
     const TP *q = use_full ? a.q : a.q_prev;
     const int pblocks = (int)((g.n + 255) / 256);
-    const int rec_words = (int)(sizeof(DepRec<TP>) / 16);
     hipError_t e;
     if (timer && (e = timer->begin(KC_DEPOSIT, 1, s)) != hipSuccess) return e;
     if (g.n > 0) {
@@ -518,8 +693,6 @@ hipError_t launch_deposit(const PushGeom &g, const SortArrays<TP> &a, bool use_f
         k_scan_final<<<nb, SCAN_THREADS, 0, s>>>(ncell, nb, bs.count, bs.blocks, bs.start);
         k_scatter<<<pblocks, 256, 0, s>>>(g.n, ncell, bs);
         k_order_cells<<<(ncell + 255) / 256, 256, 0, s>>>(ncell, bs);
-        k_sort_records<<<(int)std::min<int64_t>(8192, (g.n * rec_words + 255) / 256), 256, 0, s>>>(
-            ncell, bs, rec_words);
         k_dense_cells<TP><<<512, DENSE_THREADS, 0, s>>>(g.h, bs);
     } else {
         // Empty sort: every cell list is empty (and no cell is dense).
@@ -530,9 +703,8 @@ hipError_t launch_deposit(const PushGeom &g, const SortArrays<TP> &a, bool use_f
     }
     const int tpr = (N + DEP_TILE - 1) / DEP_TILE;
     const int tiles = tpr * tpr;
-    constexpr bool kahan = sizeof(TP) == 4;
-    k_deposit_gather<TP, TG, kahan><<<tiles, 256, 0, s>>>(N, g.h, bs.start, bs, ro, jx, jy, jz,
-                                                          bg_ro, ro1, jx1, jy1, jz1);
+    k_deposit_gather<TP, TG><<<tiles, GATHER_THREADS, 0, s>>>(N, g.h, bs.start, bs, ro, jx, jy,
+                                                              jz, bg_ro, ro1, jx1, jy1, jz1);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     if (timer) return timer->end(s);
     return hipSuccess;

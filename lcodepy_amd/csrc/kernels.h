@@ -42,33 +42,38 @@ struct BinState {
                       // sort clear them)
     int *start;       // N*N + 1
     int *blocks;      // scan block sums
-    int *key;         // per particle: cell index or -1
-    int *rank;        // per particle: its place among the particles counted into
+    // key, rank, pid and rec are indexed by binning position: the thread
+    // index j of the pass that binned the particle (the push visits particle
+    // order[j], k_bin particle j).
+    int *key;         // per position: cell index or -1
+    int *rank;        // per position: its place among the particles counted into
                       // the same cell (or among the unbinned ones, counted in
                       // count[N*N]), in arrival order
-    int *idx;         // per particle: particle ids sorted by cell, then the unbinned ones
+    int *pid;         // per position: the particle binned there
+    int *pos;         // per list slot: binning positions sorted by cell (each
+                      // cell's list ascending), then the unbinned ones
+    int *idx;         // per list slot: the particle ids of pos (the cell order
+                      // the next push of this sort visits)
     unsigned *err;    // sticky error bits
-    void *rec;        // per particle: deposit records (deposit.hip)
+    void *rec;        // per position: deposit records (deposit.hip)
     int64_t cap;      // particles the per-particle arrays hold
     int *big;         // dense cells (more than 16 particles), in queue order
     int *nbig;        // how many cells `big` holds
     int bigcap;       // capacity of `big`
     int *dslot;       // N*N: a cell's slot in `big`, or -1
     void *dense;      // per dense cell: its 5 x 5 x 4 deposit (deposit.hip)
-    void *rec_sorted; // per list position of a non-dense cell: the record of the
-                      // particle idx names there (cell order)
 };
 
 // What the deposit needs of one binned plasma particle (weights.py:35-67):
 // offsets xl, yl in [-1/2, 1/2) within its cell, dro = q / (1 - pz/gamma),
-// dro p / gamma, and the cell column iy; written at the particle's index by
-// the pass that bins it.
+// dro p / gamma, and the cell column iy; written at the particle's binning
+// position.
 template <typename T> struct alignas(16) DepRec {
     T xl, yl, c0, c1, c2, c3;
     int iy;
 };
 static_assert(sizeof(DepRec<float>) == 32 && sizeof(DepRec<double>) == 64,
-              "deposit records are copied as whole 16-byte words");
+              "deposit records are read as whole 16-byte words");
 
 // Capacity of the dense-cell queue: cells with more than 16 of the at most
 // `cap` particles.

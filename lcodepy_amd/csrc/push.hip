@@ -47,10 +47,11 @@ __device__ __forceinline__ bool reflect(double x_init, double y_init, double b, 
 }
 
 // Pushes particle k; with BIN returns its deposit cell (or -1) after writing
-// its deposit record, otherwise -1.
+// its deposit record and cell at binning position j, otherwise -1.
 template <typename TP, typename TF, bool FIRST, bool BIN>
 __device__ __forceinline__ int push_one(const PushGeom &g, const Fields6<TF> &f,
-                                        const SortArrays<TP> &a, const BinState &bs, int64_t k)
+                                        const SortArrays<TP> &a, const BinState &bs, int64_t k,
+                                        int64_t j)
 {
     const int N = g.N;
     const double b = g.boundary, dxi = g.xi_step;
@@ -89,7 +90,7 @@ __device__ __forceinline__ int push_one(const PushGeom &g, const Fields6<TF> &f,
             a.pz[k] = opz;
             a.q[k] = q;
         }
-        if (BIN) bs.key[k] = -1;
+        if (BIN) bs.key[j] = -1;
         return -1;
     }
 
@@ -192,8 +193,8 @@ __device__ __forceinline__ int push_one(const PushGeom &g, const Fields6<TF> &f,
         if (!absorbed)
             key = bin_particle<TP>(xi0 + xo, yi0 + yo, g.h, N, (double)px, (double)py,
                                    (double)pz, m, (double)q, static_cast<DepRec<TP> *>(bs.rec),
-                                   k, bs.err);
-        bs.key[k] = key;
+                                   j, bs.err);
+        bs.key[j] = key;
     }
     return key;
 }
@@ -210,13 +211,14 @@ __global__ __launch_bounds__(256) void k_push(PushGeom g, Fields6<TF> f, SortArr
     int64_t k = 0;
     if (j < g.n) {
         k = order ? (int64_t)order[j] : j;   // the particle this thread pushes
-        key = push_one<TP, TF, FIRST, BIN>(g, f, a, bs, k);
+        key = push_one<TP, TF, FIRST, BIN>(g, f, a, bs, k, j);
+        if (BIN) bs.pid[j] = (int)k;
     }
     if (BIN) {
         // Unbinned particles are counted (and ranked) in count[N * N].
         const int ck = j < g.n ? (key >= 0 ? key : g.N * g.N) : -1;
         const int r = count_rank(bs.count, ck);
-        if (j < g.n) bs.rank[k] = r;
+        if (j < g.n) bs.rank[j] = r;
     }
 }
 

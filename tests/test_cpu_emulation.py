@@ -65,6 +65,8 @@ def test_gpu_parity_suite_on_cpu_build(emu_build):
            'tests/test_gpu_kernels.py::test_push_matches_oracle[65-fp32-bulk]',
            'tests/test_gpu_kernels.py::test_push_reflects_and_absorbs_at_the_wall',
            'tests/test_gpu_kernels.py::test_plasma_step_matches_oracle[33-mobile]',
+           'tests/test_gpu_kernels.py::test_step_currents_do_not_depend_on_the_visit_order[fp64]',
+           'tests/test_gpu_kernels.py::test_step_currents_do_not_depend_on_the_visit_order[fp32-bulk]',
            'tests/test_gpu_declustering.py::test_noise_filter_matches_oracle[fp64-0.6]',
            'tests/test_gpu_declustering.py::test_noise_filter_matches_oracle[fp32-bulk-0.6]']
     env = _env()

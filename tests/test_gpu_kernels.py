@@ -414,6 +414,56 @@ def test_plasma_step_matches_oracle(N, ion_model):
 
 
 @pytest.mark.parametrize('precision', ['fp64', 'fp32-bulk'])
+def test_step_currents_do_not_depend_on_the_visit_order(precision):
+    """The currents a plasma step leaves come from its last push, which bins
+    the particles in the cell order of the deposit before (particles that
+    changed cells land out of order in their new cell's list); a standalone
+    deposit of the same particles bins them in particle order.  Both sum each
+    cell's records in particle-id order, so they agree bit for bit.  N = 33,
+    10 steps with a beam and random transverse momenta up to 0.4, so that
+    particles cross cells; the particle arrays are first shuffled (a seeded
+    permutation of the ids), so that a cell's ids do not follow the lattice
+    and a particle arriving from a neighbour lands out of id order whatever
+    the waves' timing."""
+    from lcodepy_amd.plasma3d.data import Arrays
+    from lcodepy_amd.plasma3d.initialization import init_plasma
+    from lcodepy_amd.plasma3d.solver import Plane2d3vPlasmaSolver
+    from lcodepy_amd.plasma3d.weights import get_deposit_plasma
+    N = 33
+    print(f'visit order N={N} {precision}', flush=True)
+    cfg = _cfg(N, precision=precision)
+    h = cfg.getfloat('transverse-step')
+    fields, parts, cur, const, _ = init_plasma(cfg)
+    el = parts['electrons']
+    n = el.x_init.shape[0]
+    perm = torch.from_numpy(np.random.default_rng(7).permutation(n)).to(el.x_init.device)
+    for k in el.names():
+        v = getattr(el, k)
+        if isinstance(v, torch.Tensor) and v.dim() == 1 and v.shape[0] == n:
+            setattr(el, k, v[perm].contiguous())
+    rng = np.random.default_rng(8)
+    for k in ('px', 'py'):
+        v = getattr(el, k)
+        v.copy_(torch.from_numpy(rng.uniform(-0.4, 0.4, n)).to(v.dtype))
+    solver = Plane2d3vPlasmaSolver(cfg)
+    rb_prev = np.zeros((N, N))
+    for step in range(10):
+        rb_full = _gauss_beam(N, h, -(step + 1) * 0.05, amp=-1.0)
+        parts, fields, cur = solver.step_dxi(parts, fields, cur, const, dev(rb_full), dev(rb_prev))
+        rb_prev = rb_full
+    moved = torch.abs(parts['electrons'].x_offt) > h
+    assert float(moved.double().mean()) > 0.2   # crossed at least one cell
+    single = Arrays.wrap()
+    single.sorts = {'electrons': 0}
+    out = get_deposit_plasma(cfg)(parts, single)
+    torch.cuda.synchronize()
+    solver.ctx.raise_on_errors()
+    for k, u in zip(('ro', 'jx', 'jy', 'jz'), out):
+        assert torch.any(u[0] != 0), k
+        assert torch.equal(u[0], getattr(cur, k)[0]), k
+
+
+@pytest.mark.parametrize('precision', ['fp64', 'fp32-bulk'])
 def test_energy_flux_matches_oracle(precision):
     """calculate_energy_fluxes (Strategy3D.py:6-25) on the device against the
     oracle's restatement on the same state (10 plasma steps with a beam at
