@@ -322,7 +322,8 @@ def main() -> int:
         ctx = run.solver.plasma_solver.ctx
         ctx.raise_on_errors()
         solve = {'field_solver': ctx.field_solver, 'fft_radix': ctx.fft_radix,
-                 'rfft_cap': ctx.rfft_shape[0], 'rfft_tpb': ctx.rfft_shape[1]}
+                 'rfft_cap': ctx.rfft_shape[0], 'rfft_tpb': ctx.rfft_shape[1],
+                 'rfft_wave': ctx.rfft_wave}
         results[form] = dict(elapsed=elapsed, ez=ez, n=run.n_plasma, N=run.N, kt=kt, solve=solve)
         print(f'[bench] {form}: {args.steps / elapsed:.1f} xi-steps/s, per launch '
               + ', '.join(f'{k} {v[0] * 1e6:.1f} us x{v[1]}' for k, v in kt.items()),

@@ -112,6 +112,10 @@ typedef struct lc3d_params {
     double declustering_force;
     double declustering_damping;
     double declustering_limit;
+    /* LC3D_SOLVER_RFFT: nonzero = each transform by one wave in registers,
+     * four per block (2(N-1) = 512, 1024 or 2048 only); the radix plan and
+     * the launch shape above are then not used */
+    int32_t rfft_wave;
 } lc3d_params;
 
 /* Allocates every workspace (cell bins, spectral scratch, transform
@@ -149,11 +153,12 @@ typedef struct lc3d_solve_choice {
     int32_t rfft_cap;            /* RFFT LDS size and transforms per block */
     int32_t rfft_tpb;
     double seconds;              /* device time of one lc3d_compute_fields (< 0: refused) */
+    int32_t rfft_wave;           /* RFFT by waves (lc3d_params.rfft_wave) */
 } lc3d_solve_choice;
 /* Times lc3d_compute_fields for the geometry and precision of *params on
  * synthetic inputs, `repeats` times after two warm-up calls, for the matrix
  * solve and the RFFT solve over its radix plans x LDS sizes x transforms per
- * block; writes the fastest to *best and, when table != NULL, the first
+ * block and, where the length allows, by waves; writes the fastest to *best and, when table != NULL, the first
  * table_cap candidates in the order tried (*n_tried = how many were timed).
  * Creates its own contexts and inputs and synchronises the device: a set-up
  * call, not for the xi loop.  Copy *best's fields into lc3d_params to use it. */

@@ -54,7 +54,8 @@ class Params(ctypes.Structure):
                 ('declustering_taps', ctypes.POINTER(ctypes.c_double)),
                 ('declustering_force', ctypes.c_double),
                 ('declustering_damping', ctypes.c_double),
-                ('declustering_limit', ctypes.c_double)]
+                ('declustering_limit', ctypes.c_double),
+                ('rfft_wave', ctypes.c_int32)]
 
 
 class Fields6(ctypes.Structure):
@@ -75,7 +76,8 @@ class SolveChoice(ctypes.Structure):
                 ('fft_radix', ctypes.c_int32 * 12),
                 ('rfft_cap', ctypes.c_int32),
                 ('rfft_tpb', ctypes.c_int32),
-                ('seconds', ctypes.c_double)]
+                ('seconds', ctypes.c_double),
+                ('rfft_wave', ctypes.c_int32)]
 
 
 PROBE_MAX_SOURCES = 16
@@ -186,7 +188,8 @@ class Context:
                  subtraction_coeff: float, lost_radius: float, precision: str,
                  max_particles: int, eigenvalues: np.ndarray | None = None,
                  fft_radix: tuple | None = None, field_solver: str = 'matrix',
-                 rfft_cap: int = 0, rfft_tpb: int = 0, declustering: dict | None = None):
+                 rfft_cap: int = 0, rfft_tpb: int = 0, declustering: dict | None = None,
+                 rfft_wave: bool = False):
         import torch
         if not torch.cuda.is_available():
             raise RuntimeError('lcodepy_amd needs a ROCm GPU (torch.cuda.is_available() is False)')
@@ -198,6 +201,7 @@ class Context:
         self.field_solver = field_solver
         self.fft_radix = tuple(fft_radix) if fft_radix else None
         self.rfft_shape = (int(rfft_cap), int(rfft_tpb))
+        self.rfft_wave = bool(rfft_wave)
         p = Params()
         p.grid_steps = N
         p.grid_step_size = h
@@ -212,6 +216,7 @@ class Context:
         p.field_solver = SOLVERS[field_solver]
         p.rfft_cap = int(rfft_cap)
         p.rfft_tpb = int(rfft_tpb)
+        p.rfft_wave = 1 if rfft_wave else 0
         self.declustering_side = 0
         self._dc_taps = None
         if declustering is not None:
@@ -236,7 +241,8 @@ class Context:
         if rc != 0:
             raise RuntimeError(f'lc3d_create failed with code {rc} (N={N}, precision={precision}, '
                                f'field solver {field_solver}, FFT plan {fft_radix}, rfft shape '
-                               f'{rfft_cap}/{rfft_tpb}; the "fft" solver is not built into the '
+                               f'{rfft_cap}/{rfft_tpb}, by waves {bool(rfft_wave)}; the "fft" '
+                               'solver is not built into the '
                                'GPU library)')
         self.handle = h_
 

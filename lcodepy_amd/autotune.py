@@ -48,14 +48,17 @@ def _params(N: int, h: float, xi_step: float, subtraction_coeff: float,
 def _as_dict(c: _native.SolveChoice) -> dict:
     solver = SOLVER_NAMES[c.field_solver]
     plan = tuple(c.fft_radix[i] for i in range(c.fft_nstages)) if solver == 'rfft' else None
+    wave = solver == 'rfft' and bool(c.rfft_wave)
+    if wave:
+        plan = None
     return {'field_solver': solver, 'fft_radix': plan, 'rfft_cap': int(c.rfft_cap),
-            'rfft_tpb': int(c.rfft_tpb), 'seconds': float(c.seconds)}
+            'rfft_tpb': int(c.rfft_tpb), 'rfft_wave': wave, 'seconds': float(c.seconds)}
 
 
 def tune(N: int, h: float, xi_step: float, subtraction_coeff: float, precision: str,
          repeats: int = 20, table_cap: int = 256):
     """(best, table): the fastest candidate and every candidate timed, as
-    dicts (field_solver, fft_radix, rfft_cap, rfft_tpb, seconds)."""
+    dicts (field_solver, fft_radix, rfft_cap, rfft_tpb, rfft_wave, seconds)."""
     L = _native.lib()
     p = _params(N, h, xi_step, subtraction_coeff, precision)
     best = _native.SolveChoice()
@@ -99,6 +102,8 @@ def solve_choice(device_index: int, N: int, h: float, xi_step: float, subtractio
 def _describe(c: dict) -> str:
     if c['field_solver'] == 'matrix':
         return 'matrix (MFMA transform products)'
+    if c.get('rfft_wave'):
+        return 'rfft by waves (one transform per wave in registers, 4 per block)'
     plan = 'x'.join(str(r) for r in c['fft_radix'])
     return f"rfft plan {plan}, LDS {c['rfft_cap']} complex, {c['rfft_tpb']} transforms/block"
 

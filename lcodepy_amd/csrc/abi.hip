@@ -44,7 +44,7 @@ struct lc3d_ctx {
     double *diag = nullptr;   // energy-flux partial sums, 2 DIAG_BLOCKS
     void *tw = nullptr;                                      // FFT twiddles
     uint64_t rf_stages = 0;                                  // RFFT radix plan (packed)
-    int rf_cap = 0, rf_tpb = 0;                              // RFFT launch shape
+    int rf_cap = 0, rf_tpb = 0, rf_wave = 0;                 // RFFT launch shape
     void *tdst = nullptr, *tdct = nullptr, *tdct_t = nullptr;   // transform matrices
     double *lam = nullptr;
     // declustering noise filter
@@ -211,7 +211,8 @@ SolveGeom solve_geom(const lc3d_ctx *c)
 RfftGeom rfft_geom(const lc3d_ctx *c)
 {
     return RfftGeom{c->N, c->prm.grid_step_size, c->prm.xi_step_size, c->prm.subtraction_coeff,
-                    c->rf_stages, c->rf_cap, c->rf_tpb, c->lam, c->tw, c->work, c->rowsum};
+                    c->rf_stages, c->rf_cap, c->rf_tpb, c->rf_wave, c->lam, c->tw, c->work,
+                    c->rowsum};
 }
 
 MatrixGeom matrix_geom(const lc3d_ctx *c)
@@ -444,14 +445,26 @@ int lc3d_create(const lc3d_params *p, lc3d_ctx **out)
     if (rfft) {
         const int L = 2 * M;
         const bool f32s = p->precision == LC3D_FP32_BULK;
-        c->rf_stages = p->fft_nstages > 0 ? rfft_pack_plan(L, p->fft_nstages, p->fft_radix)
-                                          : rfft_default_plan(L);
-        c->rf_cap = p->rfft_cap > 0 ? p->rfft_cap : (f32s ? 8192 : 4096);
-        c->rf_tpb = p->rfft_tpb > 0 ? p->rfft_tpb : c->rf_cap / (2 * L);
-        if (c->rf_stages == 0 || !rfft_cap_supported(f32s, c->rf_cap) || c->rf_tpb < 1 ||
-            2 * (int64_t)c->rf_tpb * L > c->rf_cap) {
-            delete c;
-            return LC3D_EINVAL;
+        if (p->rfft_wave) {
+            // One transform per wave, four per block; no radix plan or LDS size.
+            c->rf_wave = L;
+            c->rf_stages = 0;
+            c->rf_cap = 0;
+            c->rf_tpb = 4;
+            if (!rfft_wave_supported(L)) {
+                delete c;
+                return LC3D_EINVAL;
+            }
+        } else {
+            c->rf_stages = p->fft_nstages > 0 ? rfft_pack_plan(L, p->fft_nstages, p->fft_radix)
+                                              : rfft_default_plan(L);
+            c->rf_cap = p->rfft_cap > 0 ? p->rfft_cap : (f32s ? 8192 : 4096);
+            c->rf_tpb = p->rfft_tpb > 0 ? p->rfft_tpb : c->rf_cap / (2 * L);
+            if (c->rf_stages == 0 || !rfft_cap_supported(f32s, c->rf_cap) || c->rf_tpb < 1 ||
+                2 * (int64_t)c->rf_tpb * L > c->rf_cap) {
+                delete c;
+                return LC3D_EINVAL;
+            }
         }
     }
     if (fft) {

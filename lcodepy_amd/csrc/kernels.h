@@ -219,6 +219,8 @@ struct RfftGeom {
     int cap;                // LDS of a block in complex values, two buffers of cap / 2
                             // (an instantiated size)
     int tpb;                // transforms per block, tpb * L <= cap / 2
+    int wave;               // 0, or L: one transform per wave in registers (L = 512, 1024,
+                            // 2048; tpb = 4, stages and cap unused)
     const double *lam;      // N eigenvalues 4/h^2 sin^2(k pi / 2(N-1))
     const void *tw;         // L roots exp(-2 pi i t / L), complex TG
     void *work;             // 7 N*N TG
@@ -227,6 +229,8 @@ struct RfftGeom {
 
 // LDS capacities (complex values per block) the FFT kernels are built for.
 bool rfft_cap_supported(bool fp32, int cap);
+// Lengths of the FFT by waves.
+bool rfft_wave_supported(int L);
 
 template <typename TG, typename TPRED, typename TFULL, typename THALF>
 hipError_t launch_compute_fields_rfft(const RfftGeom &g, const SolveIn<TG, TPRED> &in,

@@ -197,6 +197,144 @@ template <typename T> struct RfDft<T, 16> {
     }
 };
 
+// 32 = 2 x 16: DFTs of the even and odd samples, combined with
+// exp(-2 pi i k / 32).
+template <typename T> struct RfDft<T, 32> {
+    static __device__ __forceinline__ void run(cplx_t<T> (&v)[32], const cplx_t<T> *tw, int L)
+    {
+        using X = Cx<T>;
+        // cos and sin of pi k / 16, k = 0 .. 8
+        const T c[9] = {T(1.0),
+                        T(0.98078528040323044913),
+                        T(0.92387953251128675613),
+                        T(0.83146961230254523708),
+                        T(0.70710678118654752440),
+                        T(0.55557023301960222474),
+                        T(0.38268343236508977173),
+                        T(0.19509032201612826785),
+                        T(0.0)};
+        cplx_t<T> e[16], o[16];
+#pragma unroll
+        for (int k = 0; k < 16; ++k) {
+            e[k] = v[2 * k];
+            o[k] = v[2 * k + 1];
+        }
+        RfDft<T, 16>::run(e, tw, L);
+        RfDft<T, 16>::run(o, tw, L);
+#pragma unroll
+        for (int k = 0; k < 16; ++k) {
+            // exp(-i pi k / 16) = cos - i sin; sin(pi k / 16) = cos(pi (8 - k) / 16)
+            const T cr = k <= 8 ? c[k] : -c[16 - k];
+            const T si = k <= 8 ? c[8 - k] : c[k - 8];
+            const cplx_t<T> t = X::mul_c(o[k], cr, -si);
+            v[k] = X::add(e[k], t);
+            v[k + 16] = X::sub(e[k], t);
+        }
+    }
+};
+
+// ---- one transform per wave, in registers ----------------------------------------
+// Length L = 64 P (P = 8, 16, 32 for L = 512, 1024, 2048), R3 = 64 / P.  With
+// n = 64 r + t and k = q + P k' the DFT is a radix-P DFT over r for each lane
+// t, twiddled by w_L^(t q), then for each q a 64-point DFT over t, itself a
+// radix-P DFT over s (t = u + R3 s) twiddled by w_64^(u m), and a radix-R3
+// DFT over u (k' = m + P w):
+//   stage 1  lane t        reads x[64 r + t]            writes B[q S1 + t]
+//   stage 2  lane q R3 + u reads B[q S1 + u + R3 s]     writes C[u S2 + m P + q]
+//   stage 3  lane q + P mm reads C[u S2 + m P + q]      writes X[q + P m + P^2 w]
+//            for m = mm + R3 j, j < P / R3
+// in the transform's own LDS region of L + 64 values, S1 = 64 + R3 and
+// S2 = P^2 + P, strides under which every access of a wave is spread over
+// all banks.  The block's four waves take its transforms in turn; the
+// result is in natural order in place.
+constexpr int RF_WAVE_TPB = RF_T / 64;   // transforms per block, one per wave
+template <int L> struct RfWave {
+    static constexpr int P = L / 64, R3 = 64 / P, RS = L + 64;
+    static constexpr int S1 = 64 + R3, S2 = P * P + P;
+    static_assert(P * R3 == 64 && P >= R3 && P * S1 <= RS && R3 * S2 <= RS, "wave FFT shape");
+};
+
+template <typename T, int L>
+__device__ __forceinline__ void rf_fft_wave(cplx_t<T> *lds, const cplx_t<T> *__restrict__ tw,
+                                            int ntr)
+{
+    using C = cplx_t<T>;
+    using X = Cx<T>;
+    using W = RfWave<L>;
+    constexpr int P = W::P, R3 = W::R3;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int rounds = (ntr + RF_WAVE_TPB - 1) / RF_WAVE_TPB;
+    for (int round = 0; round < rounds; ++round) {
+        const int tr = round * RF_WAVE_TPB + wave;
+        const bool act = tr < ntr;
+        C *x = lds + (act ? tr : 0) * W::RS;
+        C v[P];
+        // stage 1
+        if (act) {
+#pragma unroll
+            for (int r = 0; r < P; ++r) {
+                RF_CHECK(64 * r + lane < W::RS);
+                v[r] = x[64 * r + lane];
+            }
+            RfDft<T, P>::run(v, tw, L);
+#pragma unroll
+            for (int q = 1; q < P; ++q) v[q] = X::mul(v[q], tw[lane * q]);
+        }
+        __syncthreads();
+        if (act) {
+#pragma unroll
+            for (int q = 0; q < P; ++q) x[q * W::S1 + lane] = v[q];
+        }
+        __syncthreads();
+        // stage 2
+        const int q2 = lane / R3, u = lane % R3;
+        if (act) {
+#pragma unroll
+            for (int s = 0; s < P; ++s) {
+                RF_CHECK(q2 * W::S1 + u + R3 * s < W::RS);
+                v[s] = x[q2 * W::S1 + u + R3 * s];
+            }
+            RfDft<T, P>::run(v, tw, L);
+#pragma unroll
+            for (int m = 1; m < P; ++m) v[m] = X::mul(v[m], tw[P * u * m]);
+        }
+        __syncthreads();
+        if (act) {
+#pragma unroll
+            for (int m = 0; m < P; ++m) {
+                RF_CHECK(u * W::S2 + m * P + q2 < W::RS);
+                x[u * W::S2 + m * P + q2] = v[m];
+            }
+        }
+        __syncthreads();
+        // stage 3
+        const int q3 = lane % P, mm = lane / P;
+        if (act) {
+#pragma unroll
+            for (int j = 0; j < P / R3; ++j) {
+                C w[R3];
+#pragma unroll
+                for (int uu = 0; uu < R3; ++uu) w[uu] = x[uu * W::S2 + (mm + R3 * j) * P + q3];
+                RfDft<T, R3>::run(w, tw, L);
+#pragma unroll
+                for (int ww = 0; ww < R3; ++ww) v[j * R3 + ww] = w[ww];
+            }
+        }
+        __syncthreads();
+        if (act) {
+#pragma unroll
+            for (int j = 0; j < P / R3; ++j)
+#pragma unroll
+                for (int ww = 0; ww < R3; ++ww) {
+                    const int k = q3 + P * (mm + R3 * j) + P * P * ww;
+                    RF_CHECK(k < L);
+                    x[k] = v[j * R3 + ww];
+                }
+        }
+        __syncthreads();
+    }
+}
+
 // One Stockham stage of radix R over the ntr transforms of length L in src
 // (transform t at src[t L ..]) into dst; Ns is the product of the earlier
 // radices.  Butterfly b of a transform reads x[b + r L/R], twiddles input r
@@ -294,12 +432,35 @@ __device__ __forceinline__ void rf_block(int groups, bool fields_fastest, int &f
 
 // ---- rows, forward: right-hand sides and the transform along y ---------------
 // CAP: complex values of the block's LDS, two buffers of CAP / 2.
-template <typename TG, typename TPRED, int CAP>
+// LDS of a block: CAP complex values for the Stockham FFT, four regions of
+// WL + 64 for the FFT by waves of length WL.
+template <int CAP, int WL> struct RfLds {
+    static constexpr int size = WL > 0 ? RF_WAVE_TPB * (WL + 64) : CAP;
+};
+
+// Runs the forward FFT of the block's ntr transforms, stored from `a` (region
+// stride L, or WL + 64 for the wave FFT); returns where the result is (the
+// Stockham FFT alternates between a and b, the wave FFT works in place).
+template <typename TG, int WL>
+__device__ __forceinline__ cplx_t<TG> *rf_transform(cplx_t<TG> *a, cplx_t<TG> *b,
+                                                    const cplx_t<TG> *__restrict__ tw,
+                                                    uint64_t stages, int L, int ntr)
+{
+    if constexpr (WL > 0) {
+        rf_fft_wave<TG, WL>(a, tw, ntr);
+        return a;
+    } else {
+        return rf_fft<TG>(a, b, tw, stages, L, ntr);
+    }
+}
+
+template <typename TG, typename TPRED, int CAP, int WL>
 __global__ __launch_bounds__(RF_T) void k_rf_rows_fwd(RfftGeom g, RhsEval<TG, TPRED> rhs)
 {
     using C = cplx_t<TG>;
     using X = Cx<TG>;
-    __shared__ __align__(16) C lds[CAP];
+    constexpr int LDS = RfLds<CAP, WL>::size;
+    __shared__ __align__(16) C lds[LDS];
     const int N = g.N, M = N - 1, L = 2 * M, per = M + 1, groups = (N + 1) / 2;
     int f, grp;
     rf_block((groups + g.tpb - 1) / g.tpb, true, f, grp);
@@ -309,23 +470,24 @@ __global__ __launch_bounds__(RF_T) void k_rf_rows_fwd(RfftGeom g, RhsEval<TG, TP
     const C *tw = static_cast<const C *>(g.tw);
     TG *work = static_cast<TG *>(g.work) + (size_t)f * N * N;
     const RfDiv dper(per);
-    RF_CHECK(ntr >= 1 && 2 * ntr * L <= CAP);
+    const int RS = WL > 0 ? WL + 64 : L;   // region stride of a transform
+    RF_CHECK(ntr >= 1 && (WL > 0 ? (WL == L && ntr <= RF_WAVE_TPB) : 2 * ntr * L <= CAP));
     for (int it = threadIdx.x; it < ntr * per; it += RF_T) {
         const int tr = dper.div(it), j = it - tr * per;
         const int ia = 2 * (p0 + tr), ib = ia + 1;
         RF_CHECK(ia < N && j <= M);
         const TG a = in_rhs_domain(f, ia, j, N) ? rhs(f, ia, j) : TG(0);
         const TG b = (ib < N && in_rhs_domain(f, ib, j, N)) ? rhs(f, ib, j) : TG(0);
-        C *t = lds + tr * L;
+        C *t = lds + tr * RS;
         t[j] = X::make(a, b);
         if (j > 0 && j < M) t[L - j] = ydst ? X::make(-a, -b) : X::make(a, b);
     }
     __syncthreads();
-    const C *z = rf_fft<TG>(lds, lds + CAP / 2, tw, g.stages, L, ntr);
+    const C *z = rf_transform<TG, WL>(lds, lds + LDS / 2, tw, g.stages, L, ntr);
     for (int it = threadIdx.x; it < ntr * per; it += RF_T) {
         const int tr = dper.div(it), k = it - tr * per;
         const int ia = 2 * (p0 + tr), ib = ia + 1;
-        const C v = z[tr * L + k];
+        const C v = z[tr * RS + k];
         const bool zero = ydst && (k == 0 || k == M);
         const TG ya = zero ? TG(0) : (ydst ? -v.y : v.x);
         const TG yb = zero ? TG(0) : (ydst ? v.x : v.y);
@@ -335,12 +497,13 @@ __global__ __launch_bounds__(RF_T) void k_rf_rows_fwd(RfftGeom g, RhsEval<TG, TP
 }
 
 // ---- columns: along x, times the solver matrix, back along x -----------------
-template <typename TG, int CAP>
+template <typename TG, int CAP, int WL>
 __global__ __launch_bounds__(RF_T) void k_rf_cols(RfftGeom g)
 {
     using C = cplx_t<TG>;
     using X = Cx<TG>;
-    __shared__ __align__(16) C lds[CAP];
+    constexpr int LDS = RfLds<CAP, WL>::size;
+    __shared__ __align__(16) C lds[LDS];
     const int N = g.N, M = N - 1, L = 2 * M, per = M + 1, groups = (N + 1) / 2;
     int f, grp;
     rf_block((groups + g.tpb - 1) / g.tpb, false, f, grp);
@@ -351,44 +514,48 @@ __global__ __launch_bounds__(RF_T) void k_rf_cols(RfftGeom g)
     const bool xdst = dst_along_x(f);
     const C *tw = static_cast<const C *>(g.tw);
     TG *work = static_cast<TG *>(g.work) + (size_t)f * N * N;
-    C *a = lds, *b = lds + CAP / 2;
+    C *a = lds, *b = lds + LDS / 2;
     TG *ar = reinterpret_cast<TG *>(a);
-    RF_CHECK(ntr >= 1 && 2 * ntr * L <= CAP && j0 + nc <= N + 1);
+    const int RS = WL > 0 ? WL + 64 : L;   // region stride of a transform
+    RF_CHECK(ntr >= 1 && (WL > 0 ? (WL == L && ntr <= RF_WAVE_TPB) : 2 * ntr * L <= CAP) &&
+             j0 + nc <= N + 1);
     // Consecutive threads take consecutive columns of one row.
     for (int it = threadIdx.x; it < per * nc; it += RF_T) {
         const int i = dnc.div(it), c = it - i * nc;
         const int j = j0 + c, tr = c >> 1, part = c & 1;
         TG x = j < N ? work[(size_t)i * N + j] : TG(0);
         if (xdst && (i == 0 || i == M)) x = TG(0);
-        ar[2 * (tr * L + i) + part] = x;
-        if (i > 0 && i < M) ar[2 * (tr * L + L - i) + part] = xdst ? -x : x;
+        ar[2 * (tr * RS + i) + part] = x;
+        if (i > 0 && i < M) ar[2 * (tr * RS + L - i) + part] = xdst ? -x : x;
     }
     __syncthreads();
-    C *z = rf_fft<TG>(a, b, tw, g.stages, L, ntr);
-    C *e = z == a ? b : a;
+    C *z = rf_transform<TG, WL>(a, b, tw, g.stages, L, ntr);
+    // The wave FFT works in place: the scaled spectra overwrite it (a thread
+    // writes the k it read, and L - k > M, which no thread reads).
+    C *e = WL > 0 ? z : (z == a ? b : a);
     // Spectra times the solver matrix (zero at DST edges), written as the
     // extension the inverse transform along x starts from.
     const ScaleGeom sg{g.lam, g.c, (double)L * (double)L};
     for (int it = threadIdx.x; it < ntr * per; it += RF_T) {
         const int tr = dper.div(it), k = it - tr * per;
         const int ja = j0 + 2 * tr, jb = ja + 1;
-        const C v = z[tr * L + k];
+        const C v = z[tr * RS + k];
         const TG ya = xdst ? -v.y : v.x;
         const TG yb = xdst ? v.x : v.y;
         const double fa = solver_factor(f, k, ja, N, sg);
         const double fb = jb < N ? solver_factor(f, k, jb, N, sg) : 0.0;
         const C s = X::make((TG)((double)ya * fa), (TG)((double)yb * fb));
-        C *t = e + tr * L;
+        C *t = e + tr * RS;
         t[k] = s;
         if (k > 0 && k < M) t[L - k] = xdst ? X::neg(s) : s;
     }
     __syncthreads();
-    z = rf_fft<TG>(e, z, tw, g.stages, L, ntr);
+    z = rf_transform<TG, WL>(e, z, tw, g.stages, L, ntr);
     for (int it = threadIdx.x; it < per * nc; it += RF_T) {
         const int i = dnc.div(it), c = it - i * nc;
         const int j = j0 + c, tr = c >> 1, part = c & 1;
         if (j < N) {
-            const C v = z[tr * L + i];
+            const C v = z[tr * RS + i];
             TG x;
             if (xdst) x = (i == 0 || i == M) ? TG(0) : (part ? v.x : -v.y);
             else x = part ? v.y : v.x;
@@ -398,13 +565,14 @@ __global__ __launch_bounds__(RF_T) void k_rf_cols(RfftGeom g)
 }
 
 // ---- rows, inverse: back along y, outputs and the Bz row sums ----------------
-template <typename TG, typename TFULL, typename THALF, int CAP>
+template <typename TG, typename TFULL, typename THALF, int CAP, int WL>
 __global__ __launch_bounds__(RF_T) void k_rf_rows_inv(RfftGeom g, SolveOut<TFULL, THALF> out,
                                                       bool write_full)
 {
     using C = cplx_t<TG>;
     using X = Cx<TG>;
-    __shared__ __align__(16) C lds[CAP];
+    constexpr int LDS = RfLds<CAP, WL>::size;
+    __shared__ __align__(16) C lds[LDS];
     const int N = g.N, M = N - 1, L = 2 * M, per = M + 1, groups = (N + 1) / 2;
     int f, grp;
     rf_block((groups + g.tpb - 1) / g.tpb, true, f, grp);
@@ -414,7 +582,8 @@ __global__ __launch_bounds__(RF_T) void k_rf_rows_inv(RfftGeom g, SolveOut<TFULL
     const C *tw = static_cast<const C *>(g.tw);
     const TG *work = static_cast<const TG *>(g.work) + (size_t)f * N * N;
     const RfDiv dper(per);
-    RF_CHECK(ntr >= 1 && 2 * ntr * L <= CAP);
+    const int RS = WL > 0 ? WL + 64 : L;   // region stride of a transform
+    RF_CHECK(ntr >= 1 && (WL > 0 ? (WL == L && ntr <= RF_WAVE_TPB) : 2 * ntr * L <= CAP));
     for (int it = threadIdx.x; it < ntr * per; it += RF_T) {
         const int tr = dper.div(it), k = it - tr * per;
         const int ia = 2 * (p0 + tr), ib = ia + 1;
@@ -422,16 +591,16 @@ __global__ __launch_bounds__(RF_T) void k_rf_rows_inv(RfftGeom g, SolveOut<TFULL
         TG a = work[(size_t)ia * N + k];
         TG b = ib < N ? work[(size_t)ib * N + k] : TG(0);
         if (ydst && (k == 0 || k == M)) a = b = TG(0);
-        C *t = lds + tr * L;
+        C *t = lds + tr * RS;
         t[k] = X::make(a, b);
         if (k > 0 && k < M) t[L - k] = ydst ? X::make(-a, -b) : X::make(a, b);
     }
     __syncthreads();
-    const C *z = rf_fft<TG>(lds, lds + CAP / 2, tw, g.stages, L, ntr);
+    const C *z = rf_transform<TG, WL>(lds, lds + LDS / 2, tw, g.stages, L, ntr);
     for (int it = threadIdx.x; it < ntr * per; it += RF_T) {
         const int tr = dper.div(it), j = it - tr * per;
         const int ia = 2 * (p0 + tr), ib = ia + 1;
-        const C v = z[tr * L + j];
+        const C v = z[tr * RS + j];
         const bool zero = ydst && (j == 0 || j == M);
         const TG xa = zero ? TG(0) : (ydst ? -v.y : v.x);
         const TG xb = zero ? TG(0) : (ydst ? v.x : v.y);
@@ -446,7 +615,7 @@ __global__ __launch_bounds__(RF_T) void k_rf_rows_inv(RfftGeom g, SolveOut<TFULL
             const int tr = r >> 1, part = r & 1, row = 2 * (p0 + tr) + part;
             double s = 0;
             for (int j = lane; j <= M; j += 64) {
-                const C v = z[tr * L + j];
+                const C v = z[tr * RS + j];
                 const bool zero = ydst && (j == 0 || j == M);
                 const TG x = zero ? TG(0) : (part ? (ydst ? v.x : v.y) : (ydst ? -v.y : v.x));
                 s += (double)x;
@@ -470,15 +639,17 @@ bool rfft_cap_supported(bool fp32, int cap)
                    cap == RfCaps<double>::c2);
 }
 
-template <typename TG, typename TPRED, typename TFULL, typename THALF, int CAP>
+template <typename TG, typename TPRED, typename TFULL, typename THALF, int CAP, int WL>
 void rf_launch(const RfftGeom &g, const RhsEval<TG, TPRED> &rhs, const SolveOut<TFULL, THALF> &out,
                bool write_full, hipStream_t s)
 {
     const unsigned grid = 7 * (unsigned)(((g.N + 1) / 2 + g.tpb - 1) / g.tpb);
-    k_rf_rows_fwd<TG, TPRED, CAP><<<grid, RF_T, 0, s>>>(g, rhs);
-    k_rf_cols<TG, CAP><<<grid, RF_T, 0, s>>>(g);
-    k_rf_rows_inv<TG, TFULL, THALF, CAP><<<grid, RF_T, 0, s>>>(g, out, write_full);
+    k_rf_rows_fwd<TG, TPRED, CAP, WL><<<grid, RF_T, 0, s>>>(g, rhs);
+    k_rf_cols<TG, CAP, WL><<<grid, RF_T, 0, s>>>(g);
+    k_rf_rows_inv<TG, TFULL, THALF, CAP, WL><<<grid, RF_T, 0, s>>>(g, out, write_full);
 }
+
+bool rfft_wave_supported(int L) { return L == 512 || L == 1024 || L == 2048; }
 
 template <typename TG, typename TPRED, typename TFULL, typename THALF>
 hipError_t launch_compute_fields_rfft(const RfftGeom &g, const SolveIn<TG, TPRED> &in,
@@ -486,18 +657,25 @@ hipError_t launch_compute_fields_rfft(const RfftGeom &g, const SolveIn<TG, TPRED
                                       hipStream_t s, KernelTimer *timer)
 {
     const int N = g.N, L = 2 * (N - 1);
-    if (N < 9 || g.tpb < 1 || 2 * (int64_t)g.tpb * L > g.cap ||
-        !rfft_cap_supported(sizeof(TG) == 4, g.cap))
+    if (g.wave ? (g.wave != L || !rfft_wave_supported(L) || g.tpb != RF_WAVE_TPB)
+               : (N < 9 || g.tpb < 1 || 2 * (int64_t)g.tpb * L > g.cap ||
+                  !rfft_cap_supported(sizeof(TG) == 4, g.cap)))
         return hipErrorInvalidValue;
     const RhsEval<TG, TPRED> rhs{in, N, TG(1.0 / (2.0 * g.h)), TG(1.0 / g.xi_step), TG(g.c)};
     hipError_t e;
     if (timer && (e = timer->begin(KC_SOLVE, 3, s)) != hipSuccess) return e;
-    if (g.cap == RfCaps<TG>::c0)
-        rf_launch<TG, TPRED, TFULL, THALF, RfCaps<TG>::c0>(g, rhs, out, write_full, s);
+    if (g.wave == 512)
+        rf_launch<TG, TPRED, TFULL, THALF, 0, 512>(g, rhs, out, write_full, s);
+    else if (g.wave == 1024)
+        rf_launch<TG, TPRED, TFULL, THALF, 0, 1024>(g, rhs, out, write_full, s);
+    else if (g.wave == 2048)
+        rf_launch<TG, TPRED, TFULL, THALF, 0, 2048>(g, rhs, out, write_full, s);
+    else if (g.cap == RfCaps<TG>::c0)
+        rf_launch<TG, TPRED, TFULL, THALF, RfCaps<TG>::c0, 0>(g, rhs, out, write_full, s);
     else if (g.cap == RfCaps<TG>::c1)
-        rf_launch<TG, TPRED, TFULL, THALF, RfCaps<TG>::c1>(g, rhs, out, write_full, s);
+        rf_launch<TG, TPRED, TFULL, THALF, RfCaps<TG>::c1, 0>(g, rhs, out, write_full, s);
     else
-        rf_launch<TG, TPRED, TFULL, THALF, RfCaps<TG>::c2>(g, rhs, out, write_full, s);
+        rf_launch<TG, TPRED, TFULL, THALF, RfCaps<TG>::c2, 0>(g, rhs, out, write_full, s);
     if (timer && (e = timer->end(s)) != hipSuccess) return e;
     const int fb = (int)std::min<int64_t>(1024, ((int64_t)N * N + 255) / 256);
     k_bz_finalize<TFULL, THALF><<<fb, 256, 0, s>>>(N, g.rowsum, out.prev[F_BZ], out.full[F_BZ],

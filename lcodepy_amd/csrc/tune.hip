@@ -13,6 +13,7 @@
 #include <vector>
 
 #include "../../include/lcodepy_amd.h"
+#include "kernels.h"
 
 namespace {
 
@@ -122,6 +123,7 @@ void fill_choice(const lc3d_params &p, double sec, lc3d_solve_choice *c)
     for (int s = 0; s < 12; ++s) c->fft_radix[s] = p.fft_radix[s];
     c->rfft_cap = p.rfft_cap;
     c->rfft_tpb = p.rfft_tpb;
+    c->rfft_wave = p.rfft_wave;
     c->seconds = sec;
 }
 
@@ -142,9 +144,16 @@ extern "C" int lc3d_autotune_solve(const lc3d_params *params, int32_t repeats,
     base.max_particles = 1;   // the solve needs no particle workspaces
     base.fft_nstages = 0;
     base.rfft_cap = base.rfft_tpb = 0;
+    base.rfft_wave = 0;
     base.field_solver = LC3D_SOLVER_MATRIX;
     cands.push_back(base);
     const int L = 2 * (N - 1);
+    if (lc3d::rfft_wave_supported(L)) {
+        lc3d_params q = base;
+        q.field_solver = LC3D_SOLVER_RFFT;
+        q.rfft_wave = 1;
+        cands.push_back(q);
+    }
     const int caps32[3] = {2048, 4096, 8192}, caps64[3] = {1024, 2048, 4096};
     for (const std::vector<int> &plan : rfft_plans(L)) {
         for (int ci = 0; ci < 3; ++ci) {

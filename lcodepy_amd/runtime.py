@@ -87,12 +87,14 @@ def context(config: Config, max_particles: int | None = None,
     dev = device().index
     solver = config.field_solver()
     shape = (int(config.get('rfft-cap', 0) or 0), int(config.get('rfft-tpb', 0) or 0))
+    wave = bool(config.get('rfft-wave', False))
     if solver == 'auto':
         from . import autotune
         choice = autotune.solve_choice(dev, g['N'], g['h'], g['xi_step'],
                                        g['subtraction_coeff'], prec)
         solver, radix = choice['field_solver'], choice['fft_radix']
         shape = (choice['rfft_cap'], choice['rfft_tpb'])
+        wave = bool(choice.get('rfft_wave', False))
     decl = None
     if config.getbool('declustering-enabled'):
         from .plasma3d.noise_filter import declustering_params
@@ -100,7 +102,7 @@ def context(config: Config, max_particles: int | None = None,
         decl = declustering_params(config, side)
     dkey = None if decl is None else (decl['side'], config.getint('declustering-averaging'),
                                       decl['force'], decl['damping'], decl['limit'])
-    key = (dev, prec, radix, solver, shape, dkey) + tuple(sorted(g.items()))
+    key = (dev, prec, radix, solver, shape, wave, dkey) + tuple(sorted(g.items()))
     with _LOCK:
         ctx = None if private else _CACHE.get(key)
         if ctx is None or ctx.max_particles < cap:
@@ -108,7 +110,7 @@ def context(config: Config, max_particles: int | None = None,
                 ctx = _native.Context(eigenvalues=eigenvalues(g['N'], g['h']), precision=prec,
                                       max_particles=cap, fft_radix=radix,
                                       field_solver=solver, rfft_cap=shape[0],
-                                      rfft_tpb=shape[1], declustering=decl, **g)
+                                      rfft_tpb=shape[1], rfft_wave=wave, declustering=decl, **g)
             if not private:
                 _CACHE[key] = ctx
     return ctx

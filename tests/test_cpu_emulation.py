@@ -59,6 +59,8 @@ def test_gpu_parity_suite_on_cpu_build(emu_build):
            'tests/test_gpu_kernels.py::test_rfft_plans_and_launch_shapes[65-fp64-plan4-1024-3]',
            'tests/test_gpu_kernels.py::test_rfft_plans_and_launch_shapes[65-fp32-bulk-plan5-2048-7]',
            'tests/test_gpu_kernels.py::test_rfft_plans_and_launch_shapes[97-fp64-plan6-4096-5]',
+           'tests/test_gpu_kernels.py::test_rfft_by_waves[257-fp64]',
+           'tests/test_gpu_kernels.py::test_rfft_by_waves[257-fp32-bulk]',
            'tests/test_gpu_kernels.py::test_deposit_matches_oracle_and_is_deterministic[65-fp64]',
            'tests/test_gpu_kernels.py::test_deposit_matches_oracle_and_is_deterministic[65-fp32-bulk]',
            'tests/test_gpu_kernels.py::test_push_matches_oracle[65-fp64]',

@@ -155,6 +155,16 @@ def test_rfft_plans_and_launch_shapes(N, precision, plan, cap, tpb):
                                               'rfft-tpb': tpb})
 
 
+@pytest.mark.parametrize('N,precision', [(257, 'fp64'), (257, 'fp32-bulk'), (513, 'fp64'),
+                                         (513, 'fp32-bulk'), (1025, 'fp64'),
+                                         (1025, 'fp32-bulk')])
+def test_rfft_by_waves(N, precision):
+    """The LDS-FFT solve with each transform done by one wave in registers
+    (lengths 2(N-1) = 512, 1024, 2048: radix 8 x 8 x 8, 16 x 16 x 4,
+    32 x 32 x 2), against the oracle."""
+    test_compute_fields_matches_oracle(N, precision, 'rfft', extra={'rfft-wave': True})
+
+
 def _random_plasma(rng, p, spread=0.3, pmax=0.05):
     a = O.uniform_plasma(p)
     n = a['x_init'].shape
