@@ -206,12 +206,16 @@ __global__ __launch_bounds__(BEAM_THREADS) void k_beam_move(
     moved[k] = !is_fell && !is_lost;
 }
 
-// beam_calculator.py:124-139 (the split of a moved layer): one block per
-// group g (lost, moved, fell) copies the particles whose flag[g] is set, in
-// layer order, to the front of out_f[g] / out_i[g] and writes their number
-// to counts[g].  Chunks of the block's size: a wave's selected lanes are
-// ranked by a ballot, the waves' totals by one thread.
+// beam_calculator.py:124-139 (the split of a moved layer): for each group g
+// (lost, moved, fell; blockIdx.y) the particles whose flag[g] is set are
+// copied, in layer order, to the front of out_f[g] / out_i[g], and their
+// number is written to counts[g].  A block takes a chunk of SPLIT_THREADS
+// particles and first counts the selected particles before its chunk (at
+// most SPLIT_MAX_BLOCKS chunks, so a few KB of flags), or, launched with one
+// block per group, walks all chunks; within a chunk a wave's selected lanes
+// are ranked by a ballot, the waves' totals by one thread.
 constexpr int SPLIT_THREADS = 1024;
+constexpr int SPLIT_MAX_BLOCKS = 32;
 __global__ __launch_bounds__(SPLIT_THREADS) void k_beam_split(int64_t n, int nf, int ni,
                                                               const double *__restrict__ f,
                                                               const int64_t *__restrict__ iv,
@@ -221,15 +225,35 @@ __global__ __launch_bounds__(SPLIT_THREADS) void k_beam_split(int64_t n, int nf,
                                                               int64_t *__restrict__ counts)
 {
     __shared__ int wave_off[SPLIT_THREADS / 64 + 1];
-    const int g = blockIdx.x;
+    const int g = blockIdx.y;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const uint8_t *fl = flags + (int64_t)g * n;
     double *of = out_f + (int64_t)g * nf * n;
     int64_t *oi = out_i + (int64_t)g * ni * n;
+    const bool one = gridDim.x == 1;
+    const int64_t c_begin = one ? 0 : (int64_t)blockIdx.x * SPLIT_THREADS;
+    const int64_t c_end = one ? n : min(n, c_begin + SPLIT_THREADS);
     int64_t base = 0;
-    for (int64_t c0 = 0; c0 < n; c0 += SPLIT_THREADS) {
+    if (!one) {
+        // Selected particles before this chunk: per wave by ballots, then
+        // the waves' sums by one thread.
+        int cnt = 0;
+        for (int64_t k0 = 0; k0 < c_begin; k0 += SPLIT_THREADS)
+            cnt += __builtin_popcountll(__ballot(fl[k0 + threadIdx.x] != 0));
+        if (lane == 0) wave_off[wave] = cnt;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            int acc = 0;
+            for (int w = 0; w < SPLIT_THREADS / 64; ++w) acc += wave_off[w];
+            wave_off[SPLIT_THREADS / 64] = acc;
+        }
+        __syncthreads();
+        base = wave_off[SPLIT_THREADS / 64];
+        __syncthreads();
+    }
+    for (int64_t c0 = c_begin; c0 < c_end; c0 += SPLIT_THREADS) {
         const int64_t k = c0 + threadIdx.x;
-        const bool sel = k < n && fl[k] != 0;
+        const bool sel = k < c_end && fl[k] != 0;
         const unsigned long long m = __ballot(sel);
         const int before = __builtin_popcountll(m & ((1ull << lane) - 1ull));
         if (lane == 0) wave_off[wave] = __builtin_popcountll(m);
@@ -252,14 +276,17 @@ __global__ __launch_bounds__(SPLIT_THREADS) void k_beam_split(int64_t n, int nf,
         base += wave_off[SPLIT_THREADS / 64];
         __syncthreads();
     }
-    if (threadIdx.x == 0) counts[g] = base;
+    if (threadIdx.x == 0 && blockIdx.x == gridDim.x - 1) counts[g] = base;
 }
 
 hipError_t launch_beam_split(int64_t n, int nf, int ni, const double *f, const int64_t *iv,
                              const uint8_t *flags, double *out_f, int64_t *out_i, int64_t *counts,
                              hipStream_t s)
 {
-    k_beam_split<<<3, SPLIT_THREADS, 0, s>>>(n, nf, ni, f, iv, flags, out_f, out_i, counts);
+    const int64_t chunks = (n + SPLIT_THREADS - 1) / SPLIT_THREADS;
+    const unsigned bx = chunks >= 2 && chunks <= SPLIT_MAX_BLOCKS ? (unsigned)chunks : 1u;
+    k_beam_split<<<dim3(bx, 3), SPLIT_THREADS, 0, s>>>(n, nf, ni, f, iv, flags, out_f, out_i,
+                                                        counts);
     return hipGetLastError();
 }
 

@@ -76,28 +76,44 @@ __global__ __launch_bounds__(256) void k_bin(int64_t n, double h, int N,
 }
 
 // ---- exclusive scan of the N*N cell counts into start[0..N*N] --------------------
+// Inclusive scan of v over the wave's lanes in order (__shfl_up steps).
+__device__ __forceinline__ int wave_inclusive_scan(int v)
+{
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const int u = __shfl_up(v, d);
+        if (lane >= d) v += u;
+    }
+    return v;
+}
+
+// Exclusive scan of v over the block's SCAN_THREADS threads (every thread
+// calls it); total = the block's sum.  The waves scan themselves, then add
+// the totals of the waves before them (sh: SCAN_THREADS / 64 ints).
 __device__ __forceinline__ int block_exclusive_scan(int v, int *sh, int &total)
 {
-    // Inclusive scan over the 256 threads (Hillis-Steele in LDS).
-    const int t = threadIdx.x;
-    sh[t] = v;
+    constexpr int W = SCAN_THREADS / 64;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int incl = wave_inclusive_scan(v);
+    if (lane == 63) sh[wave] = incl;
     __syncthreads();
-    for (int off = 1; off < SCAN_THREADS; off <<= 1) {
-        const int add = t >= off ? sh[t - off] : 0;
-        __syncthreads();
-        sh[t] += add;
-        __syncthreads();
+    int off = 0, sum = 0;
+#pragma unroll
+    for (int w = 0; w < W; ++w) {
+        const int t = sh[w];
+        off += w < wave ? t : 0;
+        sum += t;
     }
-    total = sh[SCAN_THREADS - 1];
-    const int incl = sh[t];
-    __syncthreads();
-    return incl - v;
+    total = sum;
+    __syncthreads();   // sh is reused by the next call
+    return off + incl - v;
 }
 
 __global__ __launch_bounds__(SCAN_THREADS) void k_scan_blocks(int ncell, const int *__restrict__ count,
                                                               int *__restrict__ blocks)
 {
-    __shared__ int sh[SCAN_THREADS];
+    __shared__ int sh[SCAN_THREADS / 64];
     const int base = blockIdx.x * SCAN_TILE + threadIdx.x * SCAN_PER;
     int s = 0;
 #pragma unroll
@@ -115,7 +131,7 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_scan_final(int ncell, int nblo
                                                              const int *__restrict__ blocks,
                                                              int *__restrict__ start)
 {
-    __shared__ int sh[SCAN_THREADS];
+    __shared__ int sh[SCAN_THREADS / 64];
     // Offset of this tile: sum of the block totals before it.
     int pre = 0;
     for (int i = threadIdx.x; i < (int)blockIdx.x; i += SCAN_THREADS) pre += blocks[i];
@@ -693,7 +709,8 @@ hipError_t launch_deposit(const PushGeom &g, const SortArrays<TP> &a, bool use_f
         k_scan_final<<<nb, SCAN_THREADS, 0, s>>>(ncell, nb, bs.count, bs.blocks, bs.start);
         k_scatter<<<pblocks, 256, 0, s>>>(g.n, ncell, bs);
         k_order_cells<<<(ncell + 255) / 256, 256, 0, s>>>(ncell, bs);
-        k_dense_cells<TP><<<512, DENSE_THREADS, 0, s>>>(g.h, bs);
+        // One block per CU (a block takes 134 KB of LDS): the queue is striding.
+        k_dense_cells<TP><<<256, DENSE_THREADS, 0, s>>>(g.h, bs);
     } else {
         // Empty sort: every cell list is empty (and no cell is dense).
         const int nb = (ncell + 1 + SCAN_TILE - 1) / SCAN_TILE;

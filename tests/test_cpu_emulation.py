@@ -70,6 +70,9 @@ def test_gpu_parity_suite_on_cpu_build(emu_build):
            'tests/test_gpu_kernels.py::test_step_currents_do_not_depend_on_the_visit_order[fp64]',
            'tests/test_gpu_kernels.py::test_step_currents_do_not_depend_on_the_visit_order[fp32-bulk]',
            'tests/test_gpu_declustering.py::test_noise_filter_matches_oracle[fp64-0.6]',
+           'tests/test_gpu_beam.py::test_beam_split_keeps_layer_order[700]',
+           'tests/test_gpu_beam.py::test_beam_split_keeps_layer_order[5000]',
+           'tests/test_gpu_beam.py::test_beam_split_keeps_layer_order[40000]',
            'tests/test_gpu_declustering.py::test_noise_filter_matches_oracle[fp32-bulk-0.6]']
     env = _env()
     env['LD_PRELOAD'] = _libasan()

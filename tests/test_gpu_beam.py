@@ -127,3 +127,35 @@ def test_beam_move_substeps_losses_and_falls_match_oracle():
             scale = max(np.max(np.abs(o[k])), 1e-300) if o[k].size else 1.0
             assert np.max(np.abs(got - o[k]), initial=0.0) <= 1e-12 * scale, (name, k)
         assert np.array_equal(g.remaining_steps.cpu().numpy(), o['remaining_steps']), name
+
+
+@pytest.mark.parametrize('n', [1, 700, 1024, 5000, 32 * 1024, 40000])
+def test_beam_split_keeps_layer_order(n):
+    """lc3d_beam_split (beam_calculator.py:124-139) against numpy's boolean
+    gathers, bit for bit: one chunk, several chunks (one block per chunk,
+    up to 32) and more (one block walking them all); three random
+    disjoint-or-not flag rows, nine float64 and two int64 rows."""
+    from lcodepy_amd import runtime
+    from lcodepy_amd._native import ptr, stream_ptr
+    print(f'beam split n={n}', flush=True)
+    ctx = runtime.context(_config())
+    rng = np.random.default_rng(n)
+    f = rng.normal(size=(9, n))
+    iv = rng.integers(-2**40, 2**40, size=(2, n), dtype=np.int64)
+    flags = (rng.uniform(size=(3, n)) < np.array([[0.02], [0.9], [0.1]])).astype(np.uint8)
+    dev = runtime.device()
+    fd, ivd, fld = (torch.from_numpy(a).to(dev) for a in (f, iv, flags))
+    out_f = torch.full((3, 9, n), np.nan, dtype=torch.float64, device=dev)
+    out_i = torch.zeros((3, 2, n), dtype=torch.int64, device=dev)
+    counts = torch.full((3,), -1, dtype=torch.int64, device=dev)
+    ctx.check(ctx.lib.lc3d_beam_split(ctx.handle, n, 9, 2, ptr(fd), ptr(ivd), ptr(fld),
+                                      ptr(out_f), ptr(out_i), ptr(counts), stream_ptr()),
+              'lc3d_beam_split')
+    torch.cuda.synchronize()
+    got_f, got_i, got_c = out_f.cpu().numpy(), out_i.cpu().numpy(), counts.cpu().numpy()
+    for g in range(3):
+        sel = flags[g] != 0
+        m = int(sel.sum())
+        assert got_c[g] == m, (g, got_c[g], m)
+        assert np.array_equal(got_f[g][:, :m], f[:, sel]), g
+        assert np.array_equal(got_i[g][:, :m], iv[:, sel]), g
