@@ -112,9 +112,10 @@ typedef struct lc3d_params {
     double declustering_force;
     double declustering_damping;
     double declustering_limit;
-    /* LC3D_SOLVER_RFFT: nonzero = each transform by one wave in registers,
-     * four per block (2(N-1) = 512, 1024 or 2048 only); the radix plan and
-     * the launch shape above are then not used */
+    /* LC3D_SOLVER_RFFT: the kernels in which each transform is done by one
+     * wave in registers, four per block (2(N-1) = 512, 1024 or 2048 only), a
+     * mask: 1 rows forward, 2 columns, 4 rows inverse; the radix plan and the
+     * launch shape above serve the others (unused at 7) */
     int32_t rfft_wave;
 } lc3d_params;
 
@@ -153,7 +154,7 @@ typedef struct lc3d_solve_choice {
     int32_t rfft_cap;            /* RFFT LDS size and transforms per block */
     int32_t rfft_tpb;
     double seconds;              /* device time of one lc3d_compute_fields (< 0: refused) */
-    int32_t rfft_wave;           /* RFFT by waves (lc3d_params.rfft_wave) */
+    int32_t rfft_wave;           /* RFFT kernels by waves (lc3d_params.rfft_wave) */
 } lc3d_solve_choice;
 /* Times lc3d_compute_fields for the geometry and precision of *params on
  * synthetic inputs, `repeats` times after two warm-up calls, for the matrix

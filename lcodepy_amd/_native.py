@@ -189,7 +189,7 @@ class Context:
                  max_particles: int, eigenvalues: np.ndarray | None = None,
                  fft_radix: tuple | None = None, field_solver: str = 'matrix',
                  rfft_cap: int = 0, rfft_tpb: int = 0, declustering: dict | None = None,
-                 rfft_wave: bool = False):
+                 rfft_wave: int = 0):
         import torch
         if not torch.cuda.is_available():
             raise RuntimeError('lcodepy_amd needs a ROCm GPU (torch.cuda.is_available() is False)')
@@ -201,7 +201,8 @@ class Context:
         self.field_solver = field_solver
         self.fft_radix = tuple(fft_radix) if fft_radix else None
         self.rfft_shape = (int(rfft_cap), int(rfft_tpb))
-        self.rfft_wave = bool(rfft_wave)
+        # FFT kernels by waves: 1 rows forward, 2 columns, 4 rows inverse
+        self.rfft_wave = int(rfft_wave)
         p = Params()
         p.grid_steps = N
         p.grid_step_size = h
@@ -216,7 +217,7 @@ class Context:
         p.field_solver = SOLVERS[field_solver]
         p.rfft_cap = int(rfft_cap)
         p.rfft_tpb = int(rfft_tpb)
-        p.rfft_wave = 1 if rfft_wave else 0
+        p.rfft_wave = self.rfft_wave
         self.declustering_side = 0
         self._dc_taps = None
         if declustering is not None:
@@ -241,7 +242,7 @@ class Context:
         if rc != 0:
             raise RuntimeError(f'lc3d_create failed with code {rc} (N={N}, precision={precision}, '
                                f'field solver {field_solver}, FFT plan {fft_radix}, rfft shape '
-                               f'{rfft_cap}/{rfft_tpb}, by waves {bool(rfft_wave)}; the "fft" '
+                               f'{rfft_cap}/{rfft_tpb}, kernels by waves {int(rfft_wave)}; the "fft" '
                                'solver is not built into the '
                                'GPU library)')
         self.handle = h_

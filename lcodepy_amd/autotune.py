@@ -48,8 +48,8 @@ def _params(N: int, h: float, xi_step: float, subtraction_coeff: float,
 def _as_dict(c: _native.SolveChoice) -> dict:
     solver = SOLVER_NAMES[c.field_solver]
     plan = tuple(c.fft_radix[i] for i in range(c.fft_nstages)) if solver == 'rfft' else None
-    wave = solver == 'rfft' and bool(c.rfft_wave)
-    if wave:
+    wave = int(c.rfft_wave) if solver == 'rfft' else 0
+    if wave == 7:
         plan = None
     return {'field_solver': solver, 'fft_radix': plan, 'rfft_cap': int(c.rfft_cap),
             'rfft_tpb': int(c.rfft_tpb), 'rfft_wave': wave, 'seconds': float(c.seconds)}
@@ -102,10 +102,15 @@ def solve_choice(device_index: int, N: int, h: float, xi_step: float, subtractio
 def _describe(c: dict) -> str:
     if c['field_solver'] == 'matrix':
         return 'matrix (MFMA transform products)'
-    if c.get('rfft_wave'):
+    wave = int(c.get('rfft_wave') or 0)
+    if wave == 7:
         return 'rfft by waves (one transform per wave in registers, 4 per block)'
     plan = 'x'.join(str(r) for r in c['fft_radix'])
-    return f"rfft plan {plan}, LDS {c['rfft_cap']} complex, {c['rfft_tpb']} transforms/block"
+    text = f"rfft plan {plan}, LDS {c['rfft_cap']} complex, {c['rfft_tpb']} transforms/block"
+    if wave:
+        names = [k for b, k in ((1, 'rows fwd'), (2, 'cols'), (4, 'rows inv')) if wave & b]
+        text += ', by waves: ' + ' + '.join(names)
+    return text
 
 
 def main(argv=None) -> int:

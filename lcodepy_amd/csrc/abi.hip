@@ -445,16 +445,17 @@ int lc3d_create(const lc3d_params *p, lc3d_ctx **out)
     if (rfft) {
         const int L = 2 * M;
         const bool f32s = p->precision == LC3D_FP32_BULK;
-        if (p->rfft_wave) {
-            // One transform per wave, four per block; no radix plan or LDS size.
-            c->rf_wave = L;
+        // Kernels by waves (a mask of the three); the others need the radix
+        // plan and launch shape.
+        c->rf_wave = p->rfft_wave;
+        if (c->rf_wave < 0 || c->rf_wave > 7 || (c->rf_wave && !rfft_wave_supported(L))) {
+            delete c;
+            return LC3D_EINVAL;
+        }
+        if (c->rf_wave == 7) {
             c->rf_stages = 0;
             c->rf_cap = 0;
-            c->rf_tpb = 4;
-            if (!rfft_wave_supported(L)) {
-                delete c;
-                return LC3D_EINVAL;
-            }
+            c->rf_tpb = 0;
         } else {
             c->rf_stages = p->fft_nstages > 0 ? rfft_pack_plan(L, p->fft_nstages, p->fft_radix)
                                               : rfft_default_plan(L);

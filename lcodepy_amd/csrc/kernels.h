@@ -219,8 +219,10 @@ struct RfftGeom {
     int cap;                // LDS of a block in complex values, two buffers of cap / 2
                             // (an instantiated size)
     int tpb;                // transforms per block, tpb * L <= cap / 2
-    int wave;               // 0, or L: one transform per wave in registers (L = 512, 1024,
-                            // 2048; tpb = 4, stages and cap unused)
+    int wave;               // kernels by waves, one transform per wave in registers
+                            // (L = 512, 1024, 2048; four per block): bit 0 rows
+                            // forward, 1 columns, 2 rows inverse; the others use the
+                            // stages, cap and tpb above
     const double *lam;      // N eigenvalues 4/h^2 sin^2(k pi / 2(N-1))
     const void *tw;         // L roots exp(-2 pi i t / L), complex TG
     void *work;             // 7 N*N TG

@@ -463,9 +463,10 @@ __global__ __launch_bounds__(RF_T) void k_rf_rows_fwd(RfftGeom g, RhsEval<TG, TP
     __shared__ __align__(16) C lds[LDS];
     const int N = g.N, M = N - 1, L = 2 * M, per = M + 1, groups = (N + 1) / 2;
     int f, grp;
-    rf_block((groups + g.tpb - 1) / g.tpb, true, f, grp);
-    const int p0 = grp * g.tpb;
-    const int ntr = rf_block_transforms(g.tpb, groups, grp);
+    const int tpb = WL > 0 ? RF_WAVE_TPB : g.tpb;
+    rf_block((groups + tpb - 1) / tpb, true, f, grp);
+    const int p0 = grp * tpb;
+    const int ntr = rf_block_transforms(tpb, groups, grp);
     const bool ydst = dst_along_y(f);
     const C *tw = static_cast<const C *>(g.tw);
     TG *work = static_cast<TG *>(g.work) + (size_t)f * N * N;
@@ -506,9 +507,10 @@ __global__ __launch_bounds__(RF_T) void k_rf_cols(RfftGeom g)
     __shared__ __align__(16) C lds[LDS];
     const int N = g.N, M = N - 1, L = 2 * M, per = M + 1, groups = (N + 1) / 2;
     int f, grp;
-    rf_block((groups + g.tpb - 1) / g.tpb, false, f, grp);
-    const int p0 = grp * g.tpb;
-    const int ntr = rf_block_transforms(g.tpb, groups, grp);
+    const int tpb = WL > 0 ? RF_WAVE_TPB : g.tpb;
+    rf_block((groups + tpb - 1) / tpb, false, f, grp);
+    const int p0 = grp * tpb;
+    const int ntr = rf_block_transforms(tpb, groups, grp);
     const int j0 = 2 * p0, nc = 2 * ntr;   // columns j0 .. j0 + nc - 1 (the last may be N)
     const RfDiv dper(per), dnc(nc);
     const bool xdst = dst_along_x(f);
@@ -575,9 +577,10 @@ __global__ __launch_bounds__(RF_T) void k_rf_rows_inv(RfftGeom g, SolveOut<TFULL
     __shared__ __align__(16) C lds[LDS];
     const int N = g.N, M = N - 1, L = 2 * M, per = M + 1, groups = (N + 1) / 2;
     int f, grp;
-    rf_block((groups + g.tpb - 1) / g.tpb, true, f, grp);
-    const int p0 = grp * g.tpb;
-    const int ntr = rf_block_transforms(g.tpb, groups, grp);
+    const int tpb = WL > 0 ? RF_WAVE_TPB : g.tpb;
+    rf_block((groups + tpb - 1) / tpb, true, f, grp);
+    const int p0 = grp * tpb;
+    const int ntr = rf_block_transforms(tpb, groups, grp);
     const bool ydst = dst_along_y(f);
     const C *tw = static_cast<const C *>(g.tw);
     const TG *work = static_cast<const TG *>(g.work) + (size_t)f * N * N;
@@ -639,14 +642,35 @@ bool rfft_cap_supported(bool fp32, int cap)
                    cap == RfCaps<double>::c2);
 }
 
-template <typename TG, typename TPRED, typename TFULL, typename THALF, int CAP, int WL>
-void rf_launch(const RfftGeom &g, const RhsEval<TG, TPRED> &rhs, const SolveOut<TFULL, THALF> &out,
-               bool write_full, hipStream_t s)
+// Launches one of the three kernels, by waves (wave-FFT length WL) or by the
+// Stockham FFT in a block of CAP complex values.
+template <int K, typename TG, typename TPRED, typename TFULL, typename THALF, int CAP, int WL>
+void rf_kernel(const RfftGeom &g, const RhsEval<TG, TPRED> &rhs,
+               const SolveOut<TFULL, THALF> &out, bool write_full, hipStream_t s)
 {
-    const unsigned grid = 7 * (unsigned)(((g.N + 1) / 2 + g.tpb - 1) / g.tpb);
-    k_rf_rows_fwd<TG, TPRED, CAP, WL><<<grid, RF_T, 0, s>>>(g, rhs);
-    k_rf_cols<TG, CAP, WL><<<grid, RF_T, 0, s>>>(g);
-    k_rf_rows_inv<TG, TFULL, THALF, CAP, WL><<<grid, RF_T, 0, s>>>(g, out, write_full);
+    const int tpb = WL > 0 ? RF_WAVE_TPB : g.tpb;
+    const unsigned grid = 7 * (unsigned)(((g.N + 1) / 2 + tpb - 1) / tpb);
+    if constexpr (K == 0) k_rf_rows_fwd<TG, TPRED, CAP, WL><<<grid, RF_T, 0, s>>>(g, rhs);
+    else if constexpr (K == 1) k_rf_cols<TG, CAP, WL><<<grid, RF_T, 0, s>>>(g);
+    else k_rf_rows_inv<TG, TFULL, THALF, CAP, WL><<<grid, RF_T, 0, s>>>(g, out, write_full);
+}
+
+template <int K, typename TG, typename TPRED, typename TFULL, typename THALF>
+void rf_kernel_any(const RfftGeom &g, const RhsEval<TG, TPRED> &rhs,
+                   const SolveOut<TFULL, THALF> &out, bool write_full, hipStream_t s)
+{
+    const int L = 2 * (g.N - 1);
+    if (g.wave & (1 << K)) {
+        if (L == 512) rf_kernel<K, TG, TPRED, TFULL, THALF, 0, 512>(g, rhs, out, write_full, s);
+        else if (L == 1024) rf_kernel<K, TG, TPRED, TFULL, THALF, 0, 1024>(g, rhs, out, write_full, s);
+        else rf_kernel<K, TG, TPRED, TFULL, THALF, 0, 2048>(g, rhs, out, write_full, s);
+    } else if (g.cap == RfCaps<TG>::c0) {
+        rf_kernel<K, TG, TPRED, TFULL, THALF, RfCaps<TG>::c0, 0>(g, rhs, out, write_full, s);
+    } else if (g.cap == RfCaps<TG>::c1) {
+        rf_kernel<K, TG, TPRED, TFULL, THALF, RfCaps<TG>::c1, 0>(g, rhs, out, write_full, s);
+    } else {
+        rf_kernel<K, TG, TPRED, TFULL, THALF, RfCaps<TG>::c2, 0>(g, rhs, out, write_full, s);
+    }
 }
 
 bool rfft_wave_supported(int L) { return L == 512 || L == 1024 || L == 2048; }
@@ -657,25 +681,17 @@ hipError_t launch_compute_fields_rfft(const RfftGeom &g, const SolveIn<TG, TPRED
                                       hipStream_t s, KernelTimer *timer)
 {
     const int N = g.N, L = 2 * (N - 1);
-    if (g.wave ? (g.wave != L || !rfft_wave_supported(L) || g.tpb != RF_WAVE_TPB)
-               : (N < 9 || g.tpb < 1 || 2 * (int64_t)g.tpb * L > g.cap ||
-                  !rfft_cap_supported(sizeof(TG) == 4, g.cap)))
+    // A kernel not by waves needs the Stockham plan and launch shape.
+    if (g.wave < 0 || g.wave > 7 || (g.wave && !rfft_wave_supported(L)) ||
+        (g.wave != 7 && (N < 9 || g.stages == 0 || g.tpb < 1 || 2 * (int64_t)g.tpb * L > g.cap ||
+                         !rfft_cap_supported(sizeof(TG) == 4, g.cap))))
         return hipErrorInvalidValue;
     const RhsEval<TG, TPRED> rhs{in, N, TG(1.0 / (2.0 * g.h)), TG(1.0 / g.xi_step), TG(g.c)};
     hipError_t e;
     if (timer && (e = timer->begin(KC_SOLVE, 3, s)) != hipSuccess) return e;
-    if (g.wave == 512)
-        rf_launch<TG, TPRED, TFULL, THALF, 0, 512>(g, rhs, out, write_full, s);
-    else if (g.wave == 1024)
-        rf_launch<TG, TPRED, TFULL, THALF, 0, 1024>(g, rhs, out, write_full, s);
-    else if (g.wave == 2048)
-        rf_launch<TG, TPRED, TFULL, THALF, 0, 2048>(g, rhs, out, write_full, s);
-    else if (g.cap == RfCaps<TG>::c0)
-        rf_launch<TG, TPRED, TFULL, THALF, RfCaps<TG>::c0, 0>(g, rhs, out, write_full, s);
-    else if (g.cap == RfCaps<TG>::c1)
-        rf_launch<TG, TPRED, TFULL, THALF, RfCaps<TG>::c1, 0>(g, rhs, out, write_full, s);
-    else
-        rf_launch<TG, TPRED, TFULL, THALF, RfCaps<TG>::c2, 0>(g, rhs, out, write_full, s);
+    rf_kernel_any<0, TG, TPRED, TFULL, THALF>(g, rhs, out, write_full, s);
+    rf_kernel_any<1, TG, TPRED, TFULL, THALF>(g, rhs, out, write_full, s);
+    rf_kernel_any<2, TG, TPRED, TFULL, THALF>(g, rhs, out, write_full, s);
     if (timer && (e = timer->end(s)) != hipSuccess) return e;
     const int fb = (int)std::min<int64_t>(1024, ((int64_t)N * N + 255) / 256);
     k_bz_finalize<TFULL, THALF><<<fb, 256, 0, s>>>(N, g.rowsum, out.prev[F_BZ], out.full[F_BZ],

@@ -151,7 +151,7 @@ extern "C" int lc3d_autotune_solve(const lc3d_params *params, int32_t repeats,
     if (lc3d::rfft_wave_supported(L)) {
         lc3d_params q = base;
         q.field_solver = LC3D_SOLVER_RFFT;
-        q.rfft_wave = 1;
+        q.rfft_wave = 7;
         cands.push_back(q);
     }
     const int caps32[3] = {2048, 4096, 8192}, caps64[3] = {1024, 2048, 4096};
@@ -184,6 +184,25 @@ extern "C" int lc3d_autotune_solve(const lc3d_params *params, int32_t repeats,
         if (sec > 0 && (best_i < 0 || sec < best_sec)) {
             best_i = (int)i;
             best_sec = sec;
+        }
+    }
+    // Then, from the fastest Stockham form, each kernel by waves in turn,
+    // kept when it is faster.
+    if (best_i >= 0 && lc3d::rfft_wave_supported(L) && cands[best_i].field_solver == LC3D_SOLVER_RFFT &&
+        cands[best_i].rfft_wave == 0) {
+        lc3d_params cur = cands[best_i];
+        for (int k = 0; k < 3; ++k) {
+            lc3d_params q = cur;
+            q.rfft_wave |= 1 << k;
+            const double sec = time_candidate(q, in, repeats, st);
+            if (table && tried < table_cap) fill_choice(q, sec, &table[tried]);
+            ++tried;
+            cands.push_back(q);
+            if (sec > 0 && sec < best_sec) {
+                cur = q;
+                best_i = (int)cands.size() - 1;
+                best_sec = sec;
+            }
         }
     }
     if (n_tried) *n_tried = tried;

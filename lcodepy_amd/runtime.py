@@ -87,14 +87,17 @@ def context(config: Config, max_particles: int | None = None,
     dev = device().index
     solver = config.field_solver()
     shape = (int(config.get('rfft-cap', 0) or 0), int(config.get('rfft-tpb', 0) or 0))
-    wave = bool(config.get('rfft-wave', False))
+    # mask of the FFT kernels by waves (1 rows forward, 2 columns, 4 rows
+    # inverse); true = all three
+    wave = str(config.get('rfft-wave', 0) or 0).strip().lower()
+    wave = 7 if wave in ('true', 'yes', 'on') else (0 if wave in ('false', 'no', 'off') else int(wave))
     if solver == 'auto':
         from . import autotune
         choice = autotune.solve_choice(dev, g['N'], g['h'], g['xi_step'],
                                        g['subtraction_coeff'], prec)
         solver, radix = choice['field_solver'], choice['fft_radix']
         shape = (choice['rfft_cap'], choice['rfft_tpb'])
-        wave = bool(choice.get('rfft_wave', False))
+        wave = int(choice.get('rfft_wave') or 0)
     decl = None
     if config.getbool('declustering-enabled'):
         from .plasma3d.noise_filter import declustering_params

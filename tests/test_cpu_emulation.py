@@ -61,6 +61,8 @@ def test_gpu_parity_suite_on_cpu_build(emu_build):
            'tests/test_gpu_kernels.py::test_rfft_plans_and_launch_shapes[97-fp64-plan6-4096-5]',
            'tests/test_gpu_kernels.py::test_rfft_by_waves[257-fp64]',
            'tests/test_gpu_kernels.py::test_rfft_by_waves[257-fp32-bulk]',
+           'tests/test_gpu_kernels.py::test_rfft_kernels_mixed[257-fp32-bulk-3-plan2-2048-1]',
+           'tests/test_gpu_kernels.py::test_rfft_kernels_mixed[257-fp64-6-plan3-4096-3]',
            'tests/test_gpu_kernels.py::test_deposit_matches_oracle_and_is_deterministic[65-fp64]',
            'tests/test_gpu_kernels.py::test_deposit_matches_oracle_and_is_deterministic[65-fp32-bulk]',
            'tests/test_gpu_kernels.py::test_push_matches_oracle[65-fp64]',

@@ -165,6 +165,21 @@ def test_rfft_by_waves(N, precision):
     test_compute_fields_matches_oracle(N, precision, 'rfft', extra={'rfft-wave': True})
 
 
+@pytest.mark.parametrize('N,precision,mask,plan,cap,tpb', [
+    (513, 'fp32-bulk', 2, (16, 16, 4), 4096, 2),
+    (513, 'fp64', 5, (8, 8, 8, 2), 2048, 1),
+    (257, 'fp32-bulk', 3, (8, 8, 8), 2048, 1),
+    (257, 'fp64', 6, (4, 4, 4, 8), 4096, 3),
+])
+def test_rfft_kernels_mixed(N, precision, mask, plan, cap, tpb):
+    """Some of the three FFT kernels by waves (mask: 1 rows forward,
+    2 columns, 4 rows inverse), the others by the Stockham plan, against the
+    oracle: the kernels meet only in the natural-order work array."""
+    test_compute_fields_matches_oracle(N, precision, 'rfft',
+                                       extra={'rfft-wave': mask, 'fft-radix-plan': list(plan),
+                                              'rfft-cap': cap, 'rfft-tpb': tpb})
+
+
 def _random_plasma(rng, p, spread=0.3, pmax=0.05):
     a = O.uniform_plasma(p)
     n = a['x_init'].shape
