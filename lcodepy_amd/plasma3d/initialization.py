@@ -44,21 +44,25 @@ def neumann_matrix(xp, grid_steps, grid_step_size):
 
 
 def make_plasma_grid(xp, steps, step_size, fineness):
-    """1-D lattice of initial coordinates, symmetric about 0
-    (initialization.py:72-104)."""
-    plasma_step = step_size / fineness
-    if fineness % 2:
-        right_half = np.arange(steps // 2 * fineness) * plasma_step
-        left_half = -right_half[:0:-1]
-    elif 0. < fineness < 1.:
-        right_half = np.arange(steps // (2 / fineness)) * plasma_step
-        left_half = -right_half[:0:-1]
+    """Initial coordinates of the lattice along one axis, mirror-symmetric
+    about 0 (initialization.py:72-104).  With spacing h / fineness and
+    K = (steps // 2) * fineness points a side: an odd or fractional fineness
+    puts a point on the axis (the whole multiples of the spacing below K in
+    size), an even one straddles it (the odd multiples of half the spacing).  Each value
+    is formed as the reference forms it, a tick count times the spacing, so
+    the lattice is the same bit for bit."""
+    spacing = step_size / fineness
+    per_side = steps // 2 * fineness
+    if fineness % 2:   # true for a fractional fineness too (K need not be whole)
+        count = np.arange(per_side)
+        ticks = np.concatenate([-count[:0:-1], count])
     else:
-        right_half = (.5 + np.arange(steps // 2 * fineness)) * plasma_step
-        left_half = -right_half[::-1]
-    grid = np.concatenate([left_half, right_half])
-    assert np.array_equal(grid, -grid[::-1])
-    return grid
+        outer = .5 + np.arange(per_side)
+        ticks = np.concatenate([-outer[::-1], outer])
+    lattice = ticks * spacing
+    if not np.array_equal(lattice, -lattice[::-1]):
+        raise AssertionError('plasma lattice is not symmetric about the axis')
+    return lattice
 
 
 def make_plasma_single(config: Config, current_time, steps, cell_size, fineness=2):
@@ -107,8 +111,9 @@ def init_plasma(config: Config, current_time=0):
         fineness = int(fineness)
     assert grid_steps % 2 == 1
     if bound_padding_steps <= 2:
-        raise Exception("'bound_padding_steps' parameter is too low.\n"
-                        "Details: 'bound_padding_steps' must be bigger than 2. By default it is 10.")
+        raise Exception(f'bound-padding-steps = {bound_padding_steps}: at least 3 cells (10 by '
+                        'default) must separate the plasma from the window edge, so that no '
+                        'particle stencil reaches the edge cells')
     if config.getbool('dual-plasma-approach'):
         raise NotImplementedError('dual-plasma-approach is not supported in 3D '
                                   '(it is broken in the reference as well)')

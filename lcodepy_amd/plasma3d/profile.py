@@ -34,8 +34,8 @@ def _radial(time_middle, profile, width, width2, density2, r):
         coef = np.exp(-r ** 2 / 2 / width ** 2)
         coef[r > 5 * width] = 0
     else:
-        print(f'Unknown transverse distribution {profile} at t = {time_middle}, '
-              'the uniform distribution is used instead.')
+        print(f'plasma-rshape: no radial profile called {profile!r} (t = {time_middle}); '
+              'the plasma is taken as uniform')
     return coef
 
 
@@ -67,7 +67,8 @@ def _zshape_coef(config, time_middle):
                     coef = (float(seg[1]) * (float(seg[0]) - time_middle)
                             + float(seg[3]) * time_middle) / float(seg[0])
                 else:
-                    print(seg[2], 'segment of z shape is not available, a density of 1 is assumed.')
+                    print(f'plasma-zshape: segment kind {seg[2]!r} is not supported; the density '
+                          'coefficient is taken as 1')
                 break
             time_middle -= float(seg[0])
     return coef
@@ -82,6 +83,6 @@ def profile_initial_plasma(config, current_time, x_init, y_init, q, m):
     else:
         coef = _zshape_coef(config, time_middle) * _rshape_coef(config, time_middle, x_init, y_init)
     if np.any(np.asarray(coef) < 0):
-        print('A density that is < 0 is not available, a uniform density is assumed.')
+        print('plasma-shape gives a negative density somewhere; the plasma is taken as uniform')
         coef = np.ones_like(x_init)
     return q * coef, m * coef
