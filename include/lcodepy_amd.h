@@ -12,6 +12,7 @@
  *   lc3d_compute_fields   plasma3d/fields.py:276-322 (compute_fields)
  *   lc3d_plasma_step      plasma3d/solver.py:25-78   (Plane2d3vPlasmaSolver.step_dxi)
  *   lc3d_deposit_beam     beam3d/weights.py:152-163  (deposit_beam)
+ *   lc3d_layout_beam_layer beam3d/beam_calculator.py:82-93 (layout_beam_layer)
  *   lc3d_beam_substep     beam3d/beam_calculator.py:97-114 (start_moving_layer)
  *   lc3d_move_beam        beam3d/move.py:294-309     (move_beam_particles)
  *   lc3d_energy_flux      diagnostics/FXi/Strategy3D.py:6-25 (calculate_energy_fluxes)
@@ -69,6 +70,7 @@ extern "C" {
 #define LC3D_ERR_NONFINITE 2u          /* non-finite plasma position: not deposited */
 #define LC3D_ERR_BEAM_OFF_GRID 4u      /* beam stencil partly off the grid / substep cap hit */
 #define LC3D_ERR_CELL_OVERFLOW 8u      /* > 16384 particles in one cell (summed in arrival order) */
+#define LC3D_ERR_BEAM_RANGE 16u        /* beam charge contribution >= 2^40: not deposited */
 
 typedef struct lc3d_ctx lc3d_ctx;
 
@@ -243,10 +245,20 @@ int lc3d_plasma_step(lc3d_ctx *ctx, int32_t n_sorts, const lc3d_sort *sorts,
 
 /* ---- beam ----------------------------------------------------------------------- */
 /* Adds q_norm * w * xi_loc into ro_running and q_norm * w * (1 - xi_loc)
- * into ro_next for the n particles of a layer (both accumulate). */
+ * into ro_next for the n particles of a layer (both accumulate).  Each
+ * cell's sum is formed exactly (fixed point), so the result does not depend
+ * on the order of the additions. */
 int lc3d_deposit_beam(lc3d_ctx *ctx, int64_t n, int64_t plasma_layer_idx, const double *x,
                       const double *y, const double *xi, const double *q_norm,
                       double *ro_running, double *ro_next, void *stream);
+/* The layer step of BeamCalculator.layout_beam_layer in one call: deposits
+ * as lc3d_deposit_beam, then ro_running (the layer, with what the previous
+ * layer's particles left in it) becomes its finished density divided by h^2,
+ * and ro_next is overwritten with this layer's share of the next layer (its
+ * previous contents are not read). */
+int lc3d_layout_beam_layer(lc3d_ctx *ctx, int64_t n, int64_t plasma_layer_idx, const double *x,
+                           const double *y, const double *xi, const double *q_norm,
+                           double *ro_running, double *ro_next, void *stream);
 
 /* dt = mult * time_step and remaining_steps = int(1 / mult) for n particles. */
 int lc3d_beam_substep(lc3d_ctx *ctx, int64_t n, const double *q_m, const double *pz,

@@ -25,9 +25,10 @@ ERR_BITS = {
     4: 'beam particle stencil partly off the grid or substep cap reached',
     8: 'more than 16384 plasma particles in one cell (summed in arrival order, so that '
        'cell is reproducible only to rounding)',
+    16: 'beam particle charge contribution of 2^40 or more (not deposited)',
 }
 # Bits that mean results were lost; the others are reported as warnings.
-FATAL_BITS = 1 | 2 | 4
+FATAL_BITS = 1 | 2 | 4 | 16
 
 _vp = ctypes.c_void_p
 _dp = ctypes.c_void_p   # device pointers are passed as integers
@@ -104,6 +105,7 @@ class Sort(ctypes.Structure):
 SYMBOLS = ('lc3d_create', 'lc3d_destroy', 'lc3d_last_error', 'lc3d_precision',
            'lc3d_error_flags', 'lc3d_kernel_timing', 'lc3d_kernel_times', 'lc3d_move_wo_fields', 'lc3d_move_smart', 'lc3d_deposit',
            'lc3d_compute_fields', 'lc3d_plasma_step', 'lc3d_deposit_beam',
+           'lc3d_layout_beam_layer',
            'lc3d_beam_substep', 'lc3d_move_beam', 'lc3d_energy_flux', 'lc3d_probe_sample',
            'lc3d_abs_max', 'lc3d_energy_flux_noalloc', 'lc3d_autotune_solve',
            'lc3d_noise_filter', 'lc3d_beam_split')
@@ -138,6 +140,7 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
     L.lc3d_plasma_step.argtypes = [_vp, i32, P(Sort), P(Fields), P(Currents), _vp, _vp, _vp,
                                    P(Fields), P(Fields), P(Currents), _vp]
     L.lc3d_deposit_beam.argtypes = [_vp, i64, i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp]
+    L.lc3d_layout_beam_layer.argtypes = [_vp, i64, i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp]
     L.lc3d_beam_substep.argtypes = [_vp, i64, _vp, _vp, dbl, dbl, _vp, _vp, _vp]
     L.lc3d_energy_flux.argtypes = [_vp, P(Fields6), i64, _vp, _vp, _vp, _vp, _vp, _vp]
     L.lc3d_energy_flux_noalloc.argtypes = [i32, dbl, i32, P(Fields6), i64, _vp, _vp, _vp, _vp,

@@ -29,15 +29,25 @@ class BeamCalculator:
         self.rho_layout = self._zeros()
 
     def layout_beam_layer(self, beam_layer: BeamParticles, plasma_layer_idx):
-        """Deposits the layer, returns the finished density of this layer and
-        keeps the part that belongs to the next one (beam_calculator.py:82-93)."""
-        rho_layout = self._zeros()
-        if beam_layer.size != 0:
-            self.deposit(plasma_layer_idx, beam_layer.x, beam_layer.y, beam_layer.xi,
-                         beam_layer.q_norm, self.rho_layout, rho_layout)
-        self.rho_layout, rho_layout = rho_layout, self.rho_layout
-        rho_layout /= self.grid_step_size ** 2
-        return rho_layout
+        """Deposits the layer, returns the finished density of this layer
+        (divided by h^2) and keeps the part that belongs to the next one
+        (beam_calculator.py:82-93), in one library call: the running buffer
+        becomes the finished density, a fresh one receives the next layer's
+        share."""
+        import torch
+        rho_next = torch.empty(self.grid_steps, self.grid_steps, dtype=torch.float64,
+                               device=runtime.device())
+        n = beam_layer.size
+        if n:
+            beam_layer.contiguous()
+            x, y, xi, q = beam_layer.x, beam_layer.y, beam_layer.xi, beam_layer.q_norm
+        else:
+            x = y = xi = q = None
+        self.ctx.check(self.ctx.lib.lc3d_layout_beam_layer(
+            self.ctx.handle, n, int(plasma_layer_idx), ptr(x), ptr(y), ptr(xi), ptr(q),
+            ptr(self.rho_layout), ptr(rho_next), stream_ptr()), 'lc3d_layout_beam_layer')
+        finished, self.rho_layout = self.rho_layout, rho_next
+        return finished
 
     def start_moving_layer(self, beam_layer: BeamParticles, fell_size):
         """Substep multipliers for the particles that did not fall from the

@@ -42,6 +42,8 @@ struct lc3d_ctx {
     void *work = nullptr, *work2 = nullptr;
     double *rowsum = nullptr;
     double *diag = nullptr;   // energy-flux partial sums, 2 DIAG_BLOCKS
+    unsigned long long *beam_acc = nullptr;   // beam deposit fixed-point sums, 6 N*N, zero
+    float *fields32 = nullptr;   // FP32-bulk: float32 copies of six float64 fields
     void *tw = nullptr;                                      // FFT twiddles
     uint64_t rf_stages = 0;                                  // RFFT radix plan (packed)
     int rf_cap = 0, rf_tpb = 0, rf_wave = 0;                 // RFFT launch shape
@@ -364,13 +366,20 @@ const int *push_order(const lc3d_ctx *c, const lc3d_sort *s)
                : nullptr;
 }
 
+// as_f32: an FP32-bulk context pushes float64 fields rounded to float32 (the
+// step's first push, from the previous step's full fields).
 int push_sort(lc3d_ctx *c, const lc3d_sort *s, const void *const *f, bool f32, bool first,
-              bool bin, hipStream_t st)
+              bool bin, hipStream_t st, bool as_f32 = false)
 {
     const lc3d_fields6 fl{f[0], f[1], f[2], f[3], f[4], f[5]};
     hipError_t e;
     if (fp32(c)) {
-        if (f32)
+        if (!f32 && as_f32)
+            e = launch_push_f64_fields_as_f32(push_geom(c, s->n), f6<double>(&fl), c->fields32,
+                                              sort_arrays<float>(s), first, bin,
+                                              bin_state(c, s->column), push_order(c, s), st,
+                                              timer_of(c));
+        else if (f32)
             e = launch_push<float, float>(push_geom(c, s->n), f6<float>(&fl), sort_arrays<float>(s),
                                           first, bin, bin_state(c, s->column), push_order(c, s), st,
                                           timer_of(c));
@@ -517,6 +526,10 @@ int lc3d_create(const lc3d_params *p, lc3d_ctx **out)
               hipMalloc(&c->work, 7 * nn * tg) == hipSuccess &&
               hipMalloc(&c->rowsum, N * sizeof(double)) == hipSuccess &&
               hipMalloc(&c->diag, 2 * DIAG_BLOCKS * sizeof(double)) == hipSuccess &&
+              hipMalloc(&c->beam_acc, 6 * nn * sizeof(unsigned long long)) == hipSuccess &&
+              (!f32 || hipMalloc(&c->fields32, 6 * fields32_stride(N) * sizeof(float)) ==
+                           hipSuccess) &&
+              hipMemset(c->beam_acc, 0, 6 * nn * sizeof(unsigned long long)) == hipSuccess &&
               hipMalloc(&c->lam, N * sizeof(double)) == hipSuccess;
     if (ok && (fft || rfft)) ok = hipMalloc(&c->tw, 2 * (size_t)M * 2 * tg) == hipSuccess;
     if (ok && c->dc_side > 0)
@@ -621,6 +634,8 @@ void lc3d_destroy(lc3d_ctx *c)
     (void)hipFree(c->work2);
     (void)hipFree(c->rowsum);
     (void)hipFree(c->diag);
+    (void)hipFree(c->beam_acc);
+    (void)hipFree(c->fields32);
     (void)hipFree(c->tw);
     (void)hipFree(c->tdst);
     (void)hipFree(c->tdct);
@@ -775,7 +790,7 @@ int lc3d_plasma_step(lc3d_ctx *c, int32_t n_sorts, const lc3d_sort *sorts,
     int r;
     // solver.py:29-42: free stream + push 1 in fields_prev, deposit 1.
     for (int s = 0; s < n_sorts; ++s) {
-        if ((r = push_sort(c, &sorts[s], fprev, false, true, true, st))) return r;
+        if ((r = push_sort(c, &sorts[s], fprev, false, true, true, st, true))) return r;
         if ((r = deposit_sort(c, &sorts[s], true, cf, bg_ro, single, st))) return r;
     }
     // solver.py:44-47: solve 1 with fields_prev as predictor; only the half fields are kept.
@@ -816,9 +831,22 @@ int lc3d_deposit_beam(lc3d_ctx *c, int64_t n, int64_t layer, const double *x, co
     if (!c || n < 0 || !ro_running || !ro_next || (n > 0 && (!x || !y || !xi || !q_norm)))
         return fail(c, LC3D_EINVAL, "deposit_beam: bad arguments");
     const BeamGeom g{c->N, c->prm.grid_step_size, c->prm.xi_step_size, c->prm.lost_radius};
-    hipError_t e = launch_beam_deposit(g, n, layer, x, y, xi, q_norm, ro_running, ro_next, c->err,
-                                       static_cast<hipStream_t>(stream));
+    hipError_t e = launch_beam_deposit(g, n, layer, x, y, xi, q_norm, c->beam_acc, ro_running,
+                                       ro_next, c->err, static_cast<hipStream_t>(stream));
     if (e != hipSuccess) return hip_fail(c, e, "deposit_beam");
+    return LC3D_OK;
+}
+
+int lc3d_layout_beam_layer(lc3d_ctx *c, int64_t n, int64_t layer, const double *x,
+                           const double *y, const double *xi, const double *q_norm,
+                           double *ro_running, double *ro_next, void *stream)
+{
+    if (!c || n < 0 || !ro_running || !ro_next || (n > 0 && (!x || !y || !xi || !q_norm)))
+        return fail(c, LC3D_EINVAL, "layout_beam_layer: bad arguments");
+    const BeamGeom g{c->N, c->prm.grid_step_size, c->prm.xi_step_size, c->prm.lost_radius};
+    hipError_t e = launch_beam_deposit(g, n, layer, x, y, xi, q_norm, c->beam_acc, ro_running,
+                                       ro_next, c->err, static_cast<hipStream_t>(stream), true);
+    if (e != hipSuccess) return hip_fail(c, e, "layout_beam_layer");
     return LC3D_OK;
 }
 

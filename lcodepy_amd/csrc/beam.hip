@@ -4,8 +4,9 @@
 // ~10^6 plasma particles): one particle per thread would leave most of the
 // GPU idle and each thread a long chain of dependent field loads, so a group
 // of BEAM_LANES lanes serves one particle, each lane taking every
-// BEAM_LANES-th of the 25 stencil taps.  The deposit adds each tap with a
-// float64 atomic; the push (the reference's CPU explicit-midpoint scheme with
+// BEAM_LANES-th of the 25 stencil taps.  The deposit adds each tap exactly,
+// as a 128-bit fixed-point integer (below), so its result does not depend on
+// the order of the atomics; the push (the reference's CPU explicit-midpoint scheme with
 // its substep loop) sums the lanes' tap contributions by a butterfly over
 // the group, which leaves the same bits in every lane, and every lane then
 // runs the same momentum update.
@@ -18,16 +19,71 @@ constexpr int MAX_BEAM_SUBSTEPS = 1 << 16;
 constexpr int BEAM_LANES = 8;   // lanes per beam particle (a power of two dividing 64)
 constexpr int BEAM_THREADS = 256;
 
-// beam3d/weights.py:40-64 with x_h and xi_loc from :152-160.  ro0 is the
-// running layer (weight xi_loc), ro1 the next one (weight 1 - xi_loc).
+// ---- exact fixed-point accumulation ------------------------------------------------
+// A beam deposit adds each tap's charge v as the integer nearest to v 2^80,
+// split into three 42-bit limbs (bits 0-41, 42-83, 84 up, all carrying v's
+// sign), each added to its own 64-bit integer sum by an atomic whose result
+// is not waited for.  Integer sums do not depend on the order of the
+// additions, so a layer's density is the same bits whatever order the
+// atomics land in; a limb sum absorbs 2^21 full-size additions per cell
+// before it could overflow.  k_beam_finalize normalises the carries, turns
+// each cell's sum into one float64 and adds it to the layer's density.
+// Resolution 2^-80 (8e-25) of the unscaled charge; a contribution of 2^40 or
+// more is refused (ERR_BEAM_RANGE).
+constexpr int BEAM_FIX = 80, BEAM_LIMB = 42;
+constexpr double BEAM_FIX_LIMIT = 1099511627776.0;   // 2^40
+
+__device__ __forceinline__ void fixed_add(unsigned long long *l0, unsigned long long *l1,
+                                          unsigned long long *l2, double v)
+{
+    // Each step below is exact: power-of-two scalings, floors, and
+    // differences of a value and a lower multiple of the limb it stays
+    // within; the lowest limb is rounded to the nearest integer.
+    const double a = fabs(ldexp(v, BEAM_FIX));
+    const double h2 = floor(ldexp(a, -2 * BEAM_LIMB));
+    const double r2 = a - ldexp(h2, 2 * BEAM_LIMB);
+    const double h1 = floor(ldexp(r2, -BEAM_LIMB));
+    const double h0 = rint(r2 - ldexp(h1, BEAM_LIMB));
+    long long q0 = (long long)h0, q1 = (long long)h1, q2 = (long long)h2;
+    if (v < 0) {
+        q0 = -q0;
+        q1 = -q1;
+        q2 = -q2;
+    }
+    atomicAdd(l0, (unsigned long long)q0);
+    atomicAdd(l1, (unsigned long long)q1);
+    atomicAdd(l2, (unsigned long long)q2);
+}
+
+// The float64 nearest (up to two roundings) to the limb sums' value times
+// 2^-80: carries moved up so that the low limbs lie in [0, 2^42), then the
+// three exact partial values added from the top.
+__device__ __forceinline__ double fixed_value(unsigned long long l0, unsigned long long l1,
+                                              unsigned long long l2)
+{
+    long long a0 = (long long)l0, a1 = (long long)l1, a2 = (long long)l2;
+    const long long c0 = a0 >> BEAM_LIMB;   // arithmetic shift: floor division
+    a0 -= c0 << BEAM_LIMB;
+    a1 += c0;
+    const long long c1 = a1 >> BEAM_LIMB;
+    a1 -= c1 << BEAM_LIMB;
+    a2 += c1;
+    const double hi = ldexp((double)a2, 2 * BEAM_LIMB);
+    const double mid = ldexp((double)a1, BEAM_LIMB);
+    return ldexp((hi + mid) + (double)a0, -BEAM_FIX);
+}
+
+// beam3d/weights.py:40-64 with x_h and xi_loc from :152-160.  Layer 0 is the
+// running layer (weight xi_loc), layer 1 the next one (weight 1 - xi_loc);
+// acc holds the fixed-point limb sums, word w of cell o at acc[w N*N + o]
+// (w = 0-2 the running layer's limbs, 3-5 the next layer's).
 __global__ __launch_bounds__(BEAM_THREADS) void k_beam_deposit(BeamGeom g, int64_t n,
                                                                double xi_layer,
                                                                const double *__restrict__ x,
                                                                const double *__restrict__ y,
                                                                const double *__restrict__ xi,
                                                                const double *__restrict__ q_norm,
-                                                               double *__restrict__ ro0,
-                                                               double *__restrict__ ro1,
+                                                               unsigned long long *__restrict__ acc,
                                                                unsigned *err)
 {
     const int64_t t = blockIdx.x * (int64_t)BEAM_THREADS + threadIdx.x;
@@ -35,6 +91,7 @@ __global__ __launch_bounds__(BEAM_THREADS) void k_beam_deposit(BeamGeom g, int64
     const int lane = (int)(t % BEAM_LANES);
     if (k >= n) return;
     const int N = g.N;
+    const int64_t nn = (int64_t)N * N;
     const double xh = x[k] / g.h + 0.5, yh = y[k] / g.h + 0.5;
     if (!isfinite(xh) || !isfinite(yh)) {
         if (lane == 0) atomicOr(err, ERR_BEAM_OFF_GRID);
@@ -48,7 +105,7 @@ __global__ __launch_bounds__(BEAM_THREADS) void k_beam_deposit(BeamGeom g, int64
     weights4<double>(xh - fx - 0.5, wx);
     weights4<double>(yh - fy - 0.5, wy);
     const double q = q_norm[k];
-    bool off = false;
+    bool off = false, range = false;
     for (int tap = lane; tap < 25; tap += BEAM_LANES) {
         const int kx = tap / 5, ky = tap - 5 * kx;
         const int gx = ix + kx - 2, gy = iy + ky - 2;
@@ -57,10 +114,51 @@ __global__ __launch_bounds__(BEAM_THREADS) void k_beam_deposit(BeamGeom g, int64
             continue;
         }
         const double w = wx[kx] * wy[ky];
-        atomicAdd(&ro0[gx * N + gy], q * (w * xi_loc));
-        atomicAdd(&ro1[gx * N + gy], q * (w * (1 - xi_loc)));
+        const double v0 = q * (w * xi_loc), v1 = q * (w * (1 - xi_loc));
+        if (!(fabs(v0) < BEAM_FIX_LIMIT && fabs(v1) < BEAM_FIX_LIMIT)) {
+            range = true;
+            continue;
+        }
+        const int64_t o = (int64_t)gx * N + gy;
+        fixed_add(&acc[o], &acc[nn + o], &acc[2 * nn + o], v0);
+        fixed_add(&acc[3 * nn + o], &acc[4 * nn + o], &acc[5 * nn + o], v1);
     }
     if (off) atomicOr(err, ERR_BEAM_OFF_GRID);
+    if (range) atomicOr(err, ERR_BEAM_RANGE);
+}
+
+// Adds each cell's fixed-point sums to ro0 / ro1 and clears them for the
+// next deposit.  LAYOUT (beam_calculator.py:82-93): ro0 is the finished
+// layer, (ro0 + sum0) / h2, and ro1 = sum1, at every cell.
+template <bool LAYOUT>
+__global__ __launch_bounds__(256) void k_beam_finalize(int64_t nn, double h2,
+                                                       unsigned long long *__restrict__ acc,
+                                                       double *__restrict__ ro0,
+                                                       double *__restrict__ ro1)
+{
+    const int64_t o = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (o >= nn) return;
+    double sum[2] = {0.0, 0.0};
+    bool any[2] = {false, false};
+#pragma unroll
+    for (int l = 0; l < 2; ++l) {
+        unsigned long long *w = acc + 3 * l * nn + o;
+        const unsigned long long a0 = w[0], a1 = w[nn], a2 = w[2 * nn];
+        if (a0 | a1 | a2) {
+            any[l] = true;
+            sum[l] = fixed_value(a0, a1, a2);
+            w[0] = 0;
+            w[nn] = 0;
+            w[2 * nn] = 0;
+        }
+    }
+    if (LAYOUT) {
+        ro0[o] = (any[0] ? ro0[o] + sum[0] : ro0[o]) / h2;
+        ro1[o] = sum[1];
+    } else {
+        if (any[0]) ro0[o] += sum[0];
+        if (any[1]) ro1[o] += sum[1];
+    }
 }
 
 // beam3d/beam_calculator.py:18-34, 97-123: dt multiplier halved until
@@ -292,14 +390,20 @@ hipError_t launch_beam_split(int64_t n, int nf, int ni, const double *f, const i
 
 hipError_t launch_beam_deposit(const BeamGeom &g, int64_t n, int64_t plasma_layer_idx,
                                const double *x, const double *y, const double *xi,
-                               const double *q_norm, double *ro0, double *ro1, unsigned *err,
-                               hipStream_t s)
+                               const double *q_norm, unsigned long long *acc, double *ro0,
+                               double *ro1, unsigned *err, hipStream_t s, bool layout)
 {
-    if (n <= 0) return hipSuccess;
+    if (n <= 0 && !layout) return hipSuccess;
     const double xi_layer = -g.xi_step * (double)plasma_layer_idx;
-    const int64_t blocks = (n * BEAM_LANES + BEAM_THREADS - 1) / BEAM_THREADS;
-    k_beam_deposit<<<(unsigned)blocks, BEAM_THREADS, 0, s>>>(g, n, xi_layer, x, y, xi, q_norm, ro0,
-                                                             ro1, err);
+    if (n > 0) {
+        const int64_t blocks = (n * BEAM_LANES + BEAM_THREADS - 1) / BEAM_THREADS;
+        k_beam_deposit<<<(unsigned)blocks, BEAM_THREADS, 0, s>>>(g, n, xi_layer, x, y, xi, q_norm,
+                                                                 acc, err);
+    }
+    const int64_t nn = (int64_t)g.N * g.N;
+    const unsigned fb = (unsigned)((nn + 255) / 256);
+    if (layout) k_beam_finalize<true><<<fb, 256, 0, s>>>(nn, g.h * g.h, acc, ro0, ro1);
+    else k_beam_finalize<false><<<fb, 256, 0, s>>>(nn, g.h * g.h, acc, ro0, ro1);
     return hipGetLastError();
 }
 

@@ -15,6 +15,8 @@ enum : unsigned {
                                   // its substep count exceeded the cap)
     ERR_CELL_OVERFLOW = 8u,       // > 16384 plasma particles in one cell: that
                                   // cell's list is summed in arrival order
+    ERR_BEAM_RANGE = 16u,         // a beam deposit contribution of 2^40 or more in
+                                  // magnitude (not deposited)
 };
 
 template <typename T> struct cplx;

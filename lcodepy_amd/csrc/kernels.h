@@ -144,6 +144,14 @@ template <typename TP, typename TF>
 hipError_t launch_push(const PushGeom &g, const Fields6<TF> &f, const SortArrays<TP> &a,
                        bool first, bool bin, const BinState &bs, const int *order, hipStream_t s,
                        KernelTimer *timer = nullptr);
+// The FP32-bulk push from float64 fields read as float32: the six fields are
+// first rounded into f32 (6 fields32_stride(N) floats), then pushed as
+// launch_push<float, float> does, both inside the push timing.
+inline int64_t fields32_stride(int N) { return ((int64_t)N * N + 3) & ~(int64_t)3; }
+hipError_t launch_push_f64_fields_as_f32(const PushGeom &g, const Fields6<double> &f,
+                                         float *f32, const SortArrays<float> &a, bool first,
+                                         bool bin, const BinState &bs, const int *order,
+                                         hipStream_t s, KernelTimer *timer = nullptr);
 
 // plasma3d/weights.py:35-67, 249-285 for one sort.  When `binned` is false
 // the cells are computed here; otherwise the preceding push recorded them.
@@ -290,10 +298,14 @@ struct BeamGeom {
     double h, xi_step, lost_radius;
 };
 
+// acc: 6 N*N words of fixed-point sums, zero between calls (the call leaves
+// them zero).
+// With layout, ro0 becomes (ro0 + sum0) / h^2 and ro1 = sum1 (its contents
+// not read), every cell; otherwise ro0 += sum0 and ro1 += sum1.
 hipError_t launch_beam_deposit(const BeamGeom &g, int64_t n, int64_t plasma_layer_idx,
                                const double *x, const double *y, const double *xi,
-                               const double *q_norm, double *ro0, double *ro1, unsigned *err,
-                               hipStream_t s);
+                               const double *q_norm, unsigned long long *acc, double *ro0,
+                               double *ro1, unsigned *err, hipStream_t s, bool layout = false);
 
 hipError_t launch_beam_substep(int64_t n, const double *q_m, const double *pz,
                                double substepping_energy, double time_step, double *dt,

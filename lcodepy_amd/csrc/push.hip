@@ -252,6 +252,49 @@ hipError_t launch_push(const PushGeom &g, const Fields6<TF> &f, const SortArrays
     return hipSuccess;
 }
 
+// Rounds the six float64 fields into float32 copies, field f at f32 + f
+// stride (stride: N*N rounded up to a multiple of 4, so that every row of
+// four starts 16-byte aligned); four elements per thread.
+__global__ __launch_bounds__(256) void k_fields_to_f32(int64_t nn, int64_t stride,
+                                                       Fields6<double> f, float *__restrict__ out)
+{
+    const int64_t o = 4 * (blockIdx.x * (int64_t)blockDim.x + threadIdx.x);
+    if (o >= nn) return;
+    const double *src[6] = {f.Ex, f.Ey, f.Ez, f.Bx, f.By, f.Bz};
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+        float *d = out + k * stride + o;
+        if (o + 4 <= nn) {
+            const double *p = src[k] + o;
+            const float4 v = make_float4((float)p[0], (float)p[1], (float)p[2], (float)p[3]);
+            *reinterpret_cast<float4 *>(d) = v;
+        } else {
+            for (int64_t e = o; e < nn; ++e) out[k * stride + e] = (float)src[k][e];
+        }
+    }
+}
+
+hipError_t launch_push_f64_fields_as_f32(const PushGeom &g, const Fields6<double> &f,
+                                         float *f32, const SortArrays<float> &a, bool first,
+                                         bool bin, const BinState &bs, const int *order,
+                                         hipStream_t s, KernelTimer *timer)
+{
+    if (g.n <= 0) return hipSuccess;
+    const int64_t nn = (int64_t)g.N * g.N, st = fields32_stride(g.N);
+    const int blocks = (int)((g.n + 255) / 256);
+    const Fields6<float> f2{f32, f32 + st, f32 + 2 * st, f32 + 3 * st, f32 + 4 * st, f32 + 5 * st};
+    hipError_t e;
+    if (timer && (e = timer->begin(KC_PUSH, 1, s)) != hipSuccess) return e;
+    k_fields_to_f32<<<(unsigned)((nn / 4 + 256) / 256), 256, 0, s>>>(nn, st, f, f32);
+    if (first && bin) k_push<float, float, true, true><<<blocks, 256, 0, s>>>(g, f2, a, bs, order);
+    else if (first) k_push<float, float, true, false><<<blocks, 256, 0, s>>>(g, f2, a, bs, order);
+    else if (bin) k_push<float, float, false, true><<<blocks, 256, 0, s>>>(g, f2, a, bs, order);
+    else k_push<float, float, false, false><<<blocks, 256, 0, s>>>(g, f2, a, bs, order);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    if (timer) return timer->end(s);
+    return hipSuccess;
+}
+
 template hipError_t launch_move_wo_fields<double>(int64_t, double, const double *, double *,
                                                   double *, const double *, const double *,
                                                   const double *, hipStream_t);

@@ -110,6 +110,9 @@ template <typename TP, typename TG, typename TH> void run_all(int N, int prec, i
     bx[0] = 1e9; by[1] = NAN;
     double *r0 = arr<double>(nn), *r1 = arr<double>(nn);
     CHECK(lc3d_deposit_beam(r.ctx, nb, 4, bx, by, bxi, bq, r0, r1, nullptr));
+    double *r2 = arr<double>(nn);
+    CHECK(lc3d_layout_beam_layer(r.ctx, nb, 4, bx, by, bxi, bq, r1, r2, nullptr));
+    CHECK(lc3d_layout_beam_layer(r.ctx, 0, 5, nullptr, nullptr, nullptr, nullptr, r2, r1, nullptr));
     CHECK(lc3d_beam_substep(r.ctx, nb, bqm, bpz, 2.0, 1.0, bdt, bst, nullptr));
     lc3d_fields6 fk{full.Ex, full.Ey, full.Ez, full.Bx, full.By, full.Bz};
     CHECK(lc3d_move_beam(r.ctx, nb, 3, bqm, bdt, &fk, &fk, bst, bid, bx, by, bxi, bpx, bpy, bpz, l, mv, fl, nullptr));

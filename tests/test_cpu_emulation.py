@@ -75,6 +75,7 @@ def test_gpu_parity_suite_on_cpu_build(emu_build):
            'tests/test_gpu_beam.py::test_beam_split_keeps_layer_order[700]',
            'tests/test_gpu_beam.py::test_beam_split_keeps_layer_order[5000]',
            'tests/test_gpu_beam.py::test_beam_split_keeps_layer_order[40000]',
+           'tests/test_gpu_beam.py::test_beam_deposit_is_reproducible[3000]',
            'tests/test_gpu_declustering.py::test_noise_filter_matches_oracle[fp32-bulk-0.6]']
     env = _env()
     env['LD_PRELOAD'] = _libasan()
