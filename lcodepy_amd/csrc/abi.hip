@@ -25,8 +25,9 @@ struct lc3d_ctx {
     int64_t cap = 0;
     FftPlan plan{};
     // binning
-    int *count = nullptr, *start = nullptr, *blocks = nullptr, *key = nullptr, *idx = nullptr;
-    int *rank = nullptr, *pid = nullptr, *pos = nullptr;
+    int *count = nullptr, *start = nullptr, *lstart = nullptr, *base = nullptr, *key = nullptr;
+    int *idx = nullptr, *rank = nullptr, *pid = nullptr, *pos = nullptr;
+    int nbase = 0;
     unsigned *err = nullptr;
     void *rec = nullptr;
     int *big = nullptr, *nbig = nullptr, *dslot = nullptr;
@@ -162,9 +163,10 @@ PushGeom push_geom(const lc3d_ctx *c, int64_t n)
 BinState bin_state(const lc3d_ctx *c, int col)
 {
     const int64_t cap = std::max<int64_t>(1, c->cap);
-    return BinState{c->count, c->start, c->blocks, c->key,    c->rank,  c->pid,
-                    c->pos,   c->idx + col * cap, c->err, c->rec, cap,     c->big,
-                    c->nbig,  c->bigcap,          c->dslot, c->dense};
+    return BinState{c->count, c->start,  c->lstart, c->base,   c->nbase,
+                    c->key,   c->rank,   c->pid,    c->pos,    c->idx + col * cap,
+                    c->err,   c->rec,    cap,       c->big,    c->nbig,
+                    c->bigcap, c->dslot, c->dense};
 }
 
 template <typename TP> SortArrays<TP> sort_arrays(const lc3d_sort *s)
@@ -505,13 +507,15 @@ int lc3d_create(const lc3d_params *p, lc3d_ctx **out)
     c->prm.declustering_taps = nullptr;
     (void)hipGetDevice(&c->device);
     const size_t nn = (size_t)N * N;
-    const size_t nblocks = (nn + 1 + 2047) / 2048;
+    const size_t nblocks = (size_t)scan_tiles((int64_t)nn);
+    c->nbase = (int)nblocks;
     const bool f32 = p->precision == LC3D_FP32_BULK;
     const size_t tg = f32 ? sizeof(float) : sizeof(double);
     const size_t cap = (size_t)std::max<int64_t>(1, c->cap);
     bool ok = hipMalloc(&c->count, (nn + 1) * sizeof(int)) == hipSuccess &&
               hipMalloc(&c->start, (nn + 1) * sizeof(int)) == hipSuccess &&
-              hipMalloc(&c->blocks, nblocks * sizeof(int)) == hipSuccess &&
+              hipMalloc(&c->lstart, (nn + 1) * sizeof(int)) == hipSuccess &&
+              hipMalloc(&c->base, nblocks * sizeof(int)) == hipSuccess &&
               hipMalloc(&c->key, cap * sizeof(int)) == hipSuccess &&
               hipMalloc(&c->rank, cap * sizeof(int)) == hipSuccess &&
               hipMalloc(&c->idx, 2 * cap * sizeof(int)) == hipSuccess &&
@@ -598,6 +602,7 @@ int lc3d_create(const lc3d_params *p, lc3d_ctx **out)
                       hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemset(c->count, 0, (nn + 1) * sizeof(int));
     if (e == hipSuccess) e = hipMemset(c->err, 0, sizeof(unsigned));
+    if (e == hipSuccess) e = hipMemset(c->base, 0, c->nbase * sizeof(int));
     if (e == hipSuccess) e = hipMemset(c->nbig, 0, sizeof(int));
     c->bigcap = dense_cell_capacity(cap);
     if (e == hipSuccess) e = hipDeviceSynchronize();
@@ -618,7 +623,8 @@ void lc3d_destroy(lc3d_ctx *c)
     }
     (void)hipFree(c->count);
     (void)hipFree(c->start);
-    (void)hipFree(c->blocks);
+    (void)hipFree(c->lstart);
+    (void)hipFree(c->base);
     (void)hipFree(c->key);
     (void)hipFree(c->rank);
     (void)hipFree(c->idx);

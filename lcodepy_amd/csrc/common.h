@@ -149,7 +149,6 @@ __device__ __forceinline__ int bin_particle(double x, double y, double h, int N,
     r.c1 = (TP)(P * r_g);
     r.c2 = (TP)(Q * r_g);
     r.c3 = (TP)(Z * r_g);
-    r.iy = iy;
     rec[j] = r;
     return ix * N + iy;
 }

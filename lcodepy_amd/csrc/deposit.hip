@@ -44,6 +44,7 @@ constexpr int DEP_HALO = DEP_TILE + 4;
 constexpr int SCAN_THREADS = 256;
 constexpr int SCAN_PER = 8;
 constexpr int SCAN_TILE = SCAN_THREADS * SCAN_PER;
+static_assert(SCAN_TILE == SCAN_TILE_CELLS, "scan tiles as allocated");
 constexpr int SMALL_CELL = 16;          // lists sorted by their cell's thread
 constexpr int MAX_SORTED_CELL = 16384;  // lists sorted by a workgroup in LDS
 
@@ -75,7 +76,7 @@ __global__ __launch_bounds__(256) void k_bin(int64_t n, double h, int N,
     if (k < n) bs.rank[k] = r;
 }
 
-// ---- exclusive scan of the N*N cell counts into start[0..N*N] --------------------
+// ---- exclusive scan of the N*N cell counts -----------------------------------------
 // Inclusive scan of v over the wave's lanes in order (__shfl_up steps).
 __device__ __forceinline__ int wave_inclusive_scan(int v)
 {
@@ -110,52 +111,46 @@ __device__ __forceinline__ int block_exclusive_scan(int v, int *sh, int &total)
     return off + incl - v;
 }
 
-__global__ __launch_bounds__(SCAN_THREADS) void k_scan_blocks(int ncell, const int *__restrict__ count,
-                                                              int *__restrict__ blocks)
-{
-    __shared__ int sh[SCAN_THREADS / 64];
-    const int base = blockIdx.x * SCAN_TILE + threadIdx.x * SCAN_PER;
-    int s = 0;
-#pragma unroll
-    for (int i = 0; i < SCAN_PER; ++i)
-        if (base + i < ncell) s += count[base + i];
-    int total;
-    block_exclusive_scan(s, sh, total);
-    if (threadIdx.x == 0) blocks[blockIdx.x] = total;
-}
-
-// Writes start[0..ncell] and clears the counts it read (ready for the next
-// binning pass).
-__global__ __launch_bounds__(SCAN_THREADS) void k_scan_final(int ncell, int nblocks,
+// One pass, no block waiting for another.  Each block scans its tile of
+// SCAN_TILE cells into lstart (starts within the tile, lstart[ncell] included)
+// and clears the counts it read (ready for the next binning pass); then adds
+// the tile's total to base[t] of every later tile t by integer atomics, so
+// that when the pass has ended base[t] is the sum of the tiles before t,
+// whatever order the blocks ran in.  A cell's list starts at
+// lstart[c] + base[c / SCAN_TILE]: k_scatter reads it so, and k_order_cells
+// writes it to start[].  base is zero when the pass begins (the gather clears
+// it once the list sort has read it).
+__global__ __launch_bounds__(SCAN_THREADS) void k_scan_cells(int ncell, int ntiles,
                                                              int *__restrict__ count,
-                                                             const int *__restrict__ blocks,
-                                                             int *__restrict__ start)
+                                                             int *__restrict__ lstart,
+                                                             int *__restrict__ base)
 {
     __shared__ int sh[SCAN_THREADS / 64];
-    // Offset of this tile: sum of the block totals before it.
-    int pre = 0;
-    for (int i = threadIdx.x; i < (int)blockIdx.x; i += SCAN_THREADS) pre += blocks[i];
-    int dummy;
-    const int my = block_exclusive_scan(pre, sh, dummy);
-    (void)my;
-    const int offset = dummy;   // total over all threads = sum of earlier blocks
-    const int base = blockIdx.x * SCAN_TILE + threadIdx.x * SCAN_PER;
+    const int first = blockIdx.x * SCAN_TILE + threadIdx.x * SCAN_PER;
     int c[SCAN_PER];
     int s = 0;
 #pragma unroll
     for (int i = 0; i < SCAN_PER; ++i) {
-        c[i] = base + i < ncell ? count[base + i] : 0;
+        c[i] = first + i < ncell ? count[first + i] : 0;
         s += c[i];
     }
     int total;
-    int run = offset + block_exclusive_scan(s, sh, total);
+    int run = block_exclusive_scan(s, sh, total);
 #pragma unroll
     for (int i = 0; i < SCAN_PER; ++i) {
-        if (base + i <= ncell) start[base + i] = run;
-        if (base + i < ncell) count[base + i] = 0;
+        if (first + i <= ncell) lstart[first + i] = run;
+        if (first + i < ncell) count[first + i] = 0;
         run += c[i];
     }
-    (void)nblocks;
+    if (total != 0)
+        for (int t = blockIdx.x + 1 + threadIdx.x; t < ntiles; t += SCAN_THREADS)
+            atomicAdd(&base[t], total);
+}
+
+// Start of cell c's list (c <= ncell) after k_scan_cells.
+__device__ __forceinline__ int list_start(const BinState &bs, int c)
+{
+    return bs.lstart[c] + bs.base[c / SCAN_TILE];
 }
 
 // ---- scatter positions into the cell lists ---------------------------------------
@@ -170,7 +165,7 @@ __global__ __launch_bounds__(256) void k_scatter(int64_t n, int ncell, BinState 
     if (j >= n) return;
     if (j == 0) *bs.nbig = 0;   // k_order_cells refills the big-cell list
     const int key = bs.key[j];
-    const int slot = bs.start[key < 0 ? ncell : key] + bs.rank[j];
+    const int slot = list_start(bs, key < 0 ? ncell : key) + bs.rank[j];
     bs.pos[slot] = (int)j;
     bs.idx[slot] = bs.pid[j];
 }
@@ -206,7 +201,9 @@ __global__ __launch_bounds__(256) void k_order_cells(int ncell, BinState bs)
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c == 0) bs.count[ncell] = 0;   // the scatter's count of unbinned particles
     if (c >= ncell) return;
-    const int s = bs.start[c], e = bs.start[c + 1], n = e - s;
+    const int s = list_start(bs, c), e = list_start(bs, c + 1), n = e - s;
+    bs.start[c] = s;
+    if (c == ncell - 1) bs.start[ncell] = e;
     int slot = -1;
     if (n > SMALL_CELL) {
         // At most n / (SMALL_CELL + 1) < bigcap cells can be dense.
@@ -539,6 +536,10 @@ __global__ __launch_bounds__(GATHER_THREADS, GatherPlan<TP>::waves) void k_depos
     __shared__ short s_cell[HALO_CELLS];       // halo cells by particle count
     __shared__ int s_class[COST_CLASSES];
 
+    // The list sort has read the scan's tile bases: clear them for the next scan.
+    if (blockIdx.x == 0)
+        for (int i = threadIdx.x; i < bs.nbase; i += GATHER_THREADS) bs.base[i] = 0;
+
     const int tile = xcd_swizzle(blockIdx.x, gridDim.x);
     const int tpr = (N + DEP_TILE - 1) / DEP_TILE;
     const int X0 = (tile / tpr) * DEP_TILE, Y0 = (tile % tpr) * DEP_TILE;
@@ -698,24 +699,24 @@ hipError_t launch_deposit(const PushGeom &g, const SortArrays<TP> &a, bool use_f
     const double *pz = use_full ? a.pz : a.pz_prev;
     const TP *q = use_full ? a.q : a.q_prev;
     const int pblocks = (int)((g.n + 255) / 256);
+    const int ntiles = scan_tiles(ncell);   // cells 0..ncell
+    if (ntiles != bs.nbase) return hipErrorInvalidValue;
     hipError_t e;
     if (timer && (e = timer->begin(KC_DEPOSIT, 1, s)) != hipSuccess) return e;
     if (g.n > 0) {
         if (!binned)
             k_bin<TP><<<pblocks, 256, 0, s>>>(g.n, g.h, N, a.x_init, a.y_init, xo, yo, px, py, pz,
                                               q, a.m, bs);
-        const int nb = (ncell + 1 + SCAN_TILE - 1) / SCAN_TILE;
-        k_scan_blocks<<<nb, SCAN_THREADS, 0, s>>>(ncell, bs.count, bs.blocks);
-        k_scan_final<<<nb, SCAN_THREADS, 0, s>>>(ncell, nb, bs.count, bs.blocks, bs.start);
+        k_scan_cells<<<ntiles, SCAN_THREADS, 0, s>>>(ncell, ntiles, bs.count, bs.lstart,
+                                                     bs.base);
         k_scatter<<<pblocks, 256, 0, s>>>(g.n, ncell, bs);
         k_order_cells<<<(ncell + 255) / 256, 256, 0, s>>>(ncell, bs);
         // One block per CU (a block takes 134 KB of LDS): the queue is striding.
         k_dense_cells<TP><<<256, DENSE_THREADS, 0, s>>>(g.h, bs);
     } else {
         // Empty sort: every cell list is empty (and no cell is dense).
-        const int nb = (ncell + 1 + SCAN_TILE - 1) / SCAN_TILE;
-        k_scan_blocks<<<nb, SCAN_THREADS, 0, s>>>(ncell, bs.count, bs.blocks);
-        k_scan_final<<<nb, SCAN_THREADS, 0, s>>>(ncell, nb, bs.count, bs.blocks, bs.start);
+        k_scan_cells<<<ntiles, SCAN_THREADS, 0, s>>>(ncell, ntiles, bs.count, bs.lstart,
+                                                     bs.base);
         k_order_cells<<<(ncell + 255) / 256, 256, 0, s>>>(ncell, bs);
     }
     const int tpr = (N + DEP_TILE - 1) / DEP_TILE;

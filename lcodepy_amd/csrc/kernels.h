@@ -36,12 +36,19 @@ template <typename TP> struct SortArrays {
     double *pz;
 };
 
+// Tiles of the cell-count scan (2048 cells each) covering cells 0..ncell.
+constexpr int SCAN_TILE_CELLS = 2048;
+inline int scan_tiles(int64_t ncell) { return (int)((ncell + SCAN_TILE_CELLS) / SCAN_TILE_CELLS); }
+
 // Binning state shared by push and deposit (owned by the context).
 struct BinState {
     int *count;       // N*N + 1, zero between deposits (the scan and the list
                       // sort clear them)
-    int *start;       // N*N + 1
-    int *blocks;      // scan block sums
+    int *start;       // N*N + 1: each cell's list start (written by k_order_cells)
+    int *lstart;      // N*N + 1: each cell's list start within its scan tile
+    int *base;        // per scan tile: the sum of the tiles before it (zero
+                      // between deposits: the gather clears it)
+    int nbase;        // scan tiles, scan_tiles(N*N)
     // key, rank, pid and rec are indexed by binning position: the thread
     // index j of the pass that binned the particle (the push visits particle
     // order[j], k_bin particle j).
@@ -65,15 +72,15 @@ struct BinState {
 };
 
 // What the deposit needs of one binned plasma particle (weights.py:35-67):
-// offsets xl, yl in [-1/2, 1/2) within its cell, dro = q / (1 - pz/gamma),
-// dro p / gamma, and the cell column iy; written at the particle's binning
-// position.
-template <typename T> struct alignas(16) DepRec {
+// offsets xl, yl in [-1/2, 1/2) within its cell, dro = q / (1 - pz/gamma)
+// and dro p / gamma; written at the particle's binning position.  Six values
+// and no padding (24 bytes in float32, 48 in float64): every binning push
+// writes one per particle and the gather reads them all.
+template <typename T> struct alignas(2 * sizeof(T)) DepRec {
     T xl, yl, c0, c1, c2, c3;
-    int iy;
 };
-static_assert(sizeof(DepRec<float>) == 32 && sizeof(DepRec<double>) == 64,
-              "deposit records are read as whole 16-byte words");
+static_assert(sizeof(DepRec<float>) == 24 && sizeof(DepRec<double>) == 48,
+              "deposit records are six values, unpadded");
 
 // Capacity of the dense-cell queue: cells with more than 16 of the at most
 // `cap` particles.

@@ -60,6 +60,7 @@ void emu_sync_block();
 void emu_sync_wave();
 void emu_sync_group(unsigned width);
 inline void __syncthreads() { emu_sync_block(); }
+inline void __threadfence() { std::atomic_thread_fence(std::memory_order_seq_cst); }
 inline double __shfl_xor(double v, int off) {
     int t = threadIdx.x, w = t / 64;
     emu_shfl_d[t] = v;

@@ -17,7 +17,7 @@ def split_top(s):
             cur += ch
     out.append(cur.strip())
     return out
-pat = re.compile(r'([A-Za-z_][A-Za-z_0-9:]*(?:<[^;]*?>)?)\s*<<<(.*?)>>>\((.*?)\);', re.S)
+pat = re.compile(r'([A-Za-z_][A-Za-z_0-9:]*(?:<[^;(){}&]*?>)?)\s*<<<(.*?)>>>\((.*?)\);', re.S)
 def rep(m):
     name, cfg, args = m.group(1), m.group(2), m.group(3)
     c = split_top(cfg)
