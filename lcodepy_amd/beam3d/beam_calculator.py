@@ -39,7 +39,8 @@ class BeamCalculator:
                                device=runtime.device())
         n = beam_layer.size
         if n:
-            beam_layer.contiguous()
+            # Each row of a column slice of the packed tensors is itself
+            # contiguous: the rows are passed without copying the layer.
             x, y, xi, q = beam_layer.x, beam_layer.y, beam_layer.xi, beam_layer.q_norm
         else:
             x = y = xi = q = None

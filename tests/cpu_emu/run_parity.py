@@ -74,6 +74,27 @@ class _DoneEvent:
 
 
 torch.cuda.Event = _DoneEvent
+
+
+class _Stream:
+    """Every launch of the host build completes before it returns: one
+    stream, and waits between streams are no-ops."""
+    cuda_stream = 0
+
+    def __init__(self, *a, **k):
+        pass
+
+    def wait_stream(self, other):
+        pass
+
+    def wait_event(self, event):
+        pass
+
+
+torch.cuda.Stream = _Stream
+torch.cuda.current_stream = lambda *a, **k: _Stream()
+torch.cuda.stream = lambda s: _NoDevice()
+torch.Tensor.record_stream = lambda self, stream: None
 _empty = torch.empty
 
 
