@@ -4,7 +4,7 @@ from __future__ import annotations
 from .. import runtime
 from .._native import ptr, stream_ptr
 from ..config import Config
-from .data import BeamParticles
+from .data import BeamParticles, empty_rounded
 from .move import get_move_beam_particles
 from .weights import get_deposit_beam
 
@@ -80,13 +80,13 @@ class BeamCalculator:
             return PendingMove(None, None, None, None)
         beam_layer.contiguous()
         # lost, moved, fell: every entry is written by the move
-        flags = torch.empty(3, n, dtype=torch.uint8, device=dev)
+        flags = empty_rounded((3, n), torch.uint8, dev)
         self.start_moving_layer(beam_layer, fell_size)
         self.move_particles(pl_layer_idx - 1, beam_layer, fields_after_layer,
                             fields_before_layer, flags[0], flags[1], flags[2])
         nf, ni = beam_layer.f.shape[0], beam_layer.i.shape[0]
-        out_f = beam_layer.f.new_empty(3, nf, n)
-        out_i = beam_layer.i.new_empty(3, ni, n)
+        out_f = empty_rounded((3, nf, n), beam_layer.f.dtype, dev)
+        out_i = empty_rounded((3, ni, n), beam_layer.i.dtype, dev)
         counts_dev = torch.empty(3, dtype=torch.int64, device=dev)
         self.ctx.check(self.ctx.lib.lc3d_beam_split(
             self.ctx.handle, n, nf, ni, ptr(beam_layer.f), ptr(beam_layer.i), ptr(flags),

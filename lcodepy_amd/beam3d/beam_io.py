@@ -39,19 +39,24 @@ class BeamSource:
 
 
 class BeamDrain:
-    """Collects the moved layers; they are concatenated once, when read."""
+    """Collects the moved layers and concatenates them in one operation when
+    read, or once JOIN_EVERY of them are waiting: a moved layer is a view
+    into its split's output (lost, moved and fell particles of the layer),
+    which it would otherwise keep allocated."""
+
+    JOIN_EVERY = 32
 
     def __init__(self, config: Config):
         self._layers, self._lost = [], []
 
     @staticmethod
     def _joined(layers):
-        out = BeamParticles()
-        for layer in layers:
-            out.append(layer)
+        import torch
         if len(layers) > 1:
-            layers[:] = [out]
-        return out
+            f = torch.cat([layer.f for layer in layers], dim=1)
+            i = torch.cat([layer.i for layer in layers], dim=1)
+            layers[:] = [BeamParticles(f=f, i=i)]
+        return layers[0] if layers else BeamParticles()
 
     @property
     def beam_buffer(self) -> BeamParticles:
@@ -64,10 +69,14 @@ class BeamDrain:
     def push_beam_layer(self, beam_layer: BeamParticles):
         if beam_layer.size > 0:
             self._layers.append(beam_layer)
+            if len(self._layers) > self.JOIN_EVERY:
+                self._joined(self._layers)
 
     def push_beam_lost(self, lost_layer: BeamParticles):
         if lost_layer.size > 0:
             self._lost.append(lost_layer)
+            if len(self._lost) > self.JOIN_EVERY:
+                self._joined(self._lost)
 
     def beam_slice(self):
         return self.beam_buffer

@@ -17,6 +17,20 @@ INT_KEYS = ('id', 'remaining_steps')
 KEYS = FLOAT_KEYS + INT_KEYS
 
 
+def empty_rounded(shape, dtype, device):
+    """An uninitialised contiguous tensor of ``shape`` at the start of a
+    buffer whose element count is rounded up to a power of two: the layers
+    of a beam grow and shrink step by step, and with few distinct buffer
+    sizes the caching allocator reuses its blocks instead of reserving new
+    device memory (a hipMalloc in the middle of the xi loop) as they grow."""
+    import torch
+    n = 1
+    for s in shape:
+        n *= int(s)
+    cap = 1 << (n - 1).bit_length() if n > 1 else n
+    return torch.empty(cap, dtype=dtype, device=device)[:n].view(*shape)
+
+
 def _row_property(store: str, row: int):
     def get(self):
         return getattr(self, store)[row]
@@ -78,9 +92,9 @@ class BeamParticles:
         """Rows of a column slice are strided in the packed tensors; the
         kernels take contiguous rows."""
         if not self.f.is_contiguous():
-            self.f = self.f.contiguous()
+            self.f = empty_rounded(self.f.shape, self.f.dtype, self.f.device).copy_(self.f)
         if not self.i.is_contiguous():
-            self.i = self.i.contiguous()
+            self.i = empty_rounded(self.i.shape, self.i.dtype, self.i.device).copy_(self.i)
         return self
 
     def xi_sorted(self):
@@ -105,8 +119,11 @@ class BeamParticles:
         import torch
         if other.size:
             if self.size:
-                self.f = torch.cat((self.f, other.f), dim=1)
-                self.i = torch.cat((self.i, other.i), dim=1)
+                n = self.size + other.size
+                f = empty_rounded((self.f.shape[0], n), self.f.dtype, self.f.device)
+                i = empty_rounded((self.i.shape[0], n), self.i.dtype, self.i.device)
+                self.f = torch.cat((self.f, other.f), dim=1, out=f)
+                self.i = torch.cat((self.i, other.i), dim=1, out=i)
             else:
                 self.f, self.i = other.f, other.i
         return self
