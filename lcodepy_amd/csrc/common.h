@@ -2,6 +2,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 #include <cstdint>
+#include <type_traits>
 
 #include "kernels.h"
 
@@ -140,8 +141,16 @@ __device__ __forceinline__ int bin_particle(double x, double y, double h, int N,
         return -1;
     }
     const double gm = sqrt(M * M + P * P + Q * Q + Z * Z);
-    const double dro = qv / (1 - Z / gm);
-    const double r_g = dro / gm;
+    double dro, r_g;
+    if constexpr (std::is_same<TP, float>::value) {
+        // float32 records: one division by gamma, as a reciprocal
+        const double ig = 1 / gm;
+        dro = qv / (1 - Z * ig);
+        r_g = dro * ig;
+    } else {
+        dro = qv / (1 - Z / gm);
+        r_g = dro / gm;
+    }
     DepRec<TP> r;
     r.xl = (TP)xl;
     r.yl = (TP)yl;

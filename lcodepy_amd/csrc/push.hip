@@ -11,6 +11,8 @@
 #include "common.h"
 #include "kernels.h"
 
+#include <type_traits>
+
 namespace lc3d {
 
 template <typename TP>
@@ -55,6 +57,12 @@ __device__ __forceinline__ int push_one(const PushGeom &g, const Fields6<TF> &f,
 {
     const int N = g.N;
     const double b = g.boundary, dxi = g.xi_step;
+    // FP32-bulk (float32 momenta): quotients that share a divisor are taken
+    // as products with its reciprocal (one float64 division instead of two
+    // or three; the results differ by at most an ulp of float64, far below
+    // the float32 momenta).  The float64 form divides as
+    // plasma3d/move.py:52-148 does.
+    constexpr bool RCP = std::is_same<TP, float>::value;
 
     const double xi0 = a.x_init[k], yi0 = a.y_init[k];
     const double xop = a.xo_prev[k], yop = a.yo_prev[k];
@@ -72,8 +80,14 @@ __device__ __forceinline__ int push_one(const PushGeom &g, const Fields6<TF> &f,
         if (m != 0) {
             const double A = opx, B = opy, C = opz;
             const double gm = sqrt(m * m + C * C + A * A + B * B);
-            xof = xop + A / (gm - C) * dxi;
-            yof = yop + B / (gm - C) * dxi;
+            if constexpr (RCP) {
+                const double r = dxi / (gm - C);
+                xof = xop + A * r;
+                yof = yop + B * r;
+            } else {
+                xof = xop + A / (gm - C) * dxi;
+                yof = yop + B / (gm - C) * dxi;
+            }
         }
     } else {
         q = a.q[k];
@@ -142,8 +156,19 @@ __device__ __forceinline__ int push_one(const PushGeom &g, const Fields6<TF> &f,
     for (int pass = 0; pass < 2; ++pass) {
         const double P = px, Q = py, R = pz;
         const double gm = sqrt(m * m + R * R + P * P + Q * Q);
-        const TP vx = (TP)(P / gm), vy = (TP)(Q / gm), vz = (TP)(R / gm);
-        const TP factor = (TP)(qd / (1 - R / gm));
+        double ux, uy, uz;
+        if constexpr (RCP) {
+            const double ig = 1 / gm;
+            ux = P * ig;
+            uy = Q * ig;
+            uz = R * ig;
+        } else {
+            ux = P / gm;
+            uy = Q / gm;
+            uz = R / gm;
+        }
+        const TP vx = (TP)ux, vy = (TP)uy, vz = (TP)uz;
+        const TP factor = (TP)(qd / (1 - uz));
         dpx = factor * (Ex + vy * Bz - vz * By);
         dpy = factor * (Ey - vx * Bz + vz * Bx);
         dpz = factor * (Ez + vx * By - vy * Bx);
@@ -158,8 +183,14 @@ __device__ __forceinline__ int push_one(const PushGeom &g, const Fields6<TF> &f,
         const double P = px, Q = py, R = pz;
         const double gm = sqrt(m * m + R * R + P * P + Q * Q);
         const double d = gm - R;
-        xo = xop + P / d * dxi;
-        yo = yop + Q / d * dxi;
+        if constexpr (RCP) {
+            const double r = dxi / d;
+            xo = xop + P * r;
+            yo = yop + Q * r;
+        } else {
+            xo = xop + P / d * dxi;
+            yo = yop + Q / d * dxi;
+        }
     }
     px = opx + dpx;
     py = opy + dpy;
