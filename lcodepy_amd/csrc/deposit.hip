@@ -372,7 +372,7 @@ template <> struct GatherPlan<float> {
     static constexpr bool rt = false;
 };
 template <> struct GatherPlan<double> {
-    static constexpr int g0 = 1, g1 = 1, g2 = 1, g3 = 1, g4 = 1, rf = 2, waves = 4;
+    static constexpr int g0 = 1, g1 = 1, g2 = 1, g3 = 1, g4 = 1, rf = 4, waves = 4;
     static constexpr bool rt = true;   // one x offset at a time, in a runtime loop
 };
 
