@@ -237,11 +237,11 @@ def test_deposit_matches_oracle_and_is_deterministic(N, precision):
         assert not torch.any(u[1])
 
 
-@pytest.mark.parametrize('N,frac', [(65, 0.3), (129, 0.45)])
+@pytest.mark.parametrize('N,frac', [(65, 0.05), (65, 0.3), (129, 0.45)])
 def test_deposit_of_collapsed_plasma(N, frac):
-    """A fraction of the particles collapsed into the few cells around the
-    axis (thousands per cell at N=65: the workgroup sort of long cell lists;
-    over 16384 in one cell at N=129: arrival-order summation, reported as a
+    """A fraction of the particles collapsed into the cell on the axis
+    (about 800 or 4900 at N=65: the workgroup sort of long cell lists; over
+    16384 in one cell at N=129: arrival-order summation, reported as a
     warning bit).  Against the oracle within 1e-12; bitwise reproducible
     unless a cell overflows."""
     import warnings
