@@ -66,6 +66,7 @@ def test_gpu_parity_suite_on_cpu_build(emu_build):
            'tests/test_gpu_kernels.py::test_deposit_matches_oracle_and_is_deterministic[65-fp64]',
            'tests/test_gpu_kernels.py::test_deposit_matches_oracle_and_is_deterministic[65-fp32-bulk]',
            'tests/test_gpu_kernels.py::test_push_matches_oracle[65-fp64]',
+           'tests/test_gpu_kernels.py::test_deposit_of_collapsed_plasma[65-0.005]',
            'tests/test_gpu_kernels.py::test_deposit_of_collapsed_plasma[65-0.05]',
            'tests/test_gpu_kernels.py::test_deposit_of_collapsed_plasma[65-0.3]',
            'tests/test_gpu_kernels.py::test_push_matches_oracle[65-fp32-bulk]',
