@@ -179,9 +179,17 @@ def ptr(t) -> int | None:
 
 
 def stream_ptr(stream=None) -> int:
+    """The HIP stream handle of ``stream`` or of torch's current stream.  The
+    current stream is read with torch's raw-stream query (one C call, where
+    ``torch.cuda.current_stream()`` re-checks the device on every call, a
+    sizeable part of an xi step's host time)."""
     import torch
-    s = torch.cuda.current_stream() if stream is None else stream
-    return s.cuda_stream
+    if stream is not None:
+        return stream.cuda_stream
+    raw = getattr(torch._C, '_cuda_getCurrentRawStream', None)
+    if raw is None:
+        return torch.cuda.current_stream().cuda_stream
+    return raw(torch._C._cuda_getDevice())
 
 
 class Context:

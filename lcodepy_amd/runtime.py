@@ -19,11 +19,19 @@ _CACHE: dict = {}
 _LOCK = threading.Lock()
 
 
+_HAVE_GPU = False
+
+
 def device():
+    """torch's current device; raises without a GPU (no CPU fallback).  The
+    availability check runs until it first succeeds (it is slow)."""
+    global _HAVE_GPU
     import torch
-    if not torch.cuda.is_available():
-        raise RuntimeError('lcodepy_amd needs a ROCm GPU: torch.cuda.is_available() is False '
-                           '(the HIP path has no CPU fallback)')
+    if not _HAVE_GPU:
+        if not torch.cuda.is_available():
+            raise RuntimeError('lcodepy_amd needs a ROCm GPU: torch.cuda.is_available() is '
+                               'False (the HIP path has no CPU fallback)')
+        _HAVE_GPU = True
     return torch.device('cuda', torch.cuda.current_device())
 
 
