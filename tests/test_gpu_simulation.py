@@ -158,3 +158,33 @@ def test_beam_evolution_matches_oracle():
     print('drained beam and last on-axis Ez vs oracle:', {k: f'{v:.1e}' for k, v in errs.items()})
     for k, v in errs.items():
         assert v < 1e-9, (k, v)
+
+
+def test_evolving_beam_runs_are_bitwise_equal():
+    """Two runs of the same evolving-beam case (the beam layers deposited and
+    moved on their own stream beside the plasma steps, tensors shared between
+    the streams by events and record_stream) give the same bits: the final
+    plasma fields and currents and the drained beam.  A missed wait between
+    the streams or a buffer reused while the other stream still reads it
+    would show up as a difference."""
+    from lcodepy_amd.simulation import Simulation
+    cfg = fixture_config('beam_evol')
+    cfg.update({'window-width': 3.2, 'transverse-step': 0.05, 'window-length': 4,
+                'time-limit': 2, 'time-step': 1})
+    beam_params = {'current': 0.5, 'particles_in_layer': 3000}
+    runs = []
+    for _ in range(2):
+        sim = Simulation(config=cfg, beam_parameters=beam_params, runas_filename='')
+        sim.push_solver.verbose = False
+        sim.step()
+        _, fields, currents = sim.push_solver._plasmastate
+        drained = sim.beam_drain.beam_slice()
+        out = {k: getattr(fields, k).cpu() for k in ('Ex', 'Ey', 'Ez', 'Bx', 'By', 'Bz')}
+        out.update({k: getattr(currents, k).cpu() for k in ('ro', 'jx', 'jy', 'jz')})
+        out.update({'beam_' + k: getattr(drained, k).cpu() for k in ('id', 'x', 'y', 'xi',
+                                                                     'px', 'py', 'pz')})
+        runs.append(out)
+    print(f'evolving beam: {runs[0]["beam_id"].numel()} drained particles, two runs')
+    assert runs[0]['beam_id'].numel() > 0
+    for k in runs[0]:
+        assert torch.equal(runs[0][k], runs[1][k]), k
